@@ -1,0 +1,148 @@
+#!/usr/bin/env python3
+"""Benchmark: residual + Jacobian assembly of the nonlinear Q2 thermal problem.
+
+Workload (one GPU): [0,1]^3, 96^3 Q2 hexahedra (193^3 = 7,189,057 DOFs,
+454,756,609 CRS nonzeros), 3x3x3 Gauss points, k(T,x) = k0_e (1 + T^2) with
+k0_e = exp(0.5 z_e) (z_e ~ N(0,1), seed 2024), source f_e ~ U[0,1] (seed 2025),
+T = 0 on x = 0 and x = 1.  A step is one full J+R assembly (zero-fill, eight
+colour launches of the fused kernel, Dirichlet rows) with u resident in HBM.
+
+Prints one JSON line: assembly MDOF/s plus the roofline of the dominant kernel
+(algorithmic bytes B_asm per launch / its mean HIP-event duration) and a CPU
+baseline (the oracle, single thread, on a 24^3 sample scaled per element).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def c2_problem(n: int):
+    from mrhyde_amd.abi import MHX_RNG_LOGNORMAL, MHX_RNG_UNIFORM, Problem, rng_fill
+
+    ne = n ** 3
+    k0 = rng_fill(2024, ne, MHX_RNG_LOGNORMAL, 0.0, 0.5)
+    f = rng_fill(2025, ne, MHX_RNG_UNIFORM, 0.0, 1.0)
+    return Problem(dim=3, order=2, quadrature=4, n=(n, n, n), k0=k0, src_elem=f, diff_u2=1.0,
+                   dbc_sides=1 | 2)
+
+
+def b_asm(ne: int, n_e: int, n_c: int, n_vert: int, dim: int) -> float:
+    """Algorithmic bytes of one coloured J+R assembly (element-matrix read+write,
+    u gather, residual read+write, coefficients, vertex coordinates)."""
+    return ne * (16 * n_e * n_e + 24 * n_e + 8 * n_c) + 8 * dim * n_vert
+
+
+def cpu_baseline(n_full: int, sample_n: int = 24):
+    from mrhyde_amd.abi import rng_fill
+    from oracle import Oracle
+
+    p = c2_problem(sample_n)
+    o = Oracle(p)
+    u = rng_fill(7, o.n_dof, 0, 0.0, 1.0)
+    o.setDirichlet(u)
+    t0 = time.perf_counter()
+    o.assembleJacRes(u)
+    dt = time.perf_counter() - t0
+    ne_s, ne_f = sample_n ** 3, n_full ** 3
+    n_full_dof = (2 * n_full + 1) ** 3
+    t_full = dt * ne_f / ne_s
+    return {"value": n_full_dof / t_full / 1e6, "unit": "MDOF/s", "cores": 1, "kind": "port",
+            "sample": f"oracle J+R assembly of the same problem on a {sample_n}^3 Q2 mesh ({ne_s} elements, "
+                      f"{dt:.2f} s), scaled per element to {n_full}^3"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=96, help="elements per axis")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
+              "(each GPU would own a slab of elements; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from mrhyde_amd import Mhx
+    from mrhyde_amd.abi import MHX_BUF_U, rng_fill
+
+    n = args.n
+    prob = c2_problem(n)
+    t0 = time.perf_counter()
+    g = Mhx(prob)
+    t_setup = time.perf_counter() - t0
+    u = rng_fill(7, g.n_dof, 0, 0.0, 1.0)
+    g.setDirichlet(u)
+    g.state_upload(MHX_BUF_U, u)
+    print(f"[bench] setup {t_setup:.2f}s n_dof={g.n_dof} nnz={g.nnz} n_elem={g.n_elem}", file=sys.stderr, flush=True)
+
+    for _ in range(args.warmup):
+        g.assemble_resident()
+    g.synchronize()
+    g.profile(True)
+    g.profile_reset()
+    g.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        g.assemble_resident()
+    g.synchronize()
+    dt = time.perf_counter() - t0
+    prof = g.profile_read()
+    g.profile(False)
+    ms_step = dt * 1e3 / args.steps
+    mdofs = g.n_dof / (ms_step * 1e-3) / 1e6
+
+    kname = "k_thermal<jacres>"
+    k_ms, k_cnt = prof.get(kname, (float("nan"), 0))
+    launches_per_step = k_cnt / args.steps
+    mean_launch_ms = k_ms / max(k_cnt, 1)
+    n_vert = (n + 1) ** 3
+    B = b_asm(g.n_elem, 27, 2, n_vert, 3)
+    B_launch = B / launches_per_step
+    achieved = B_launch / (mean_launch_ms * 1e-3) / 1e9
+    B_min = 8 * g.nnz + 16 * g.n_dof + 8 * 2 * g.n_elem + 8 * 3 * n_vert
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(n)
+
+    out = {
+        "metric": "assembly MDOF/s (% HBM peak); subgrid solves/s; p50/p99 solve latency @1e-10",
+        "value": round(mdofs, 2),
+        "unit": "MDOF/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded lognormal k0, uniform source, random u)",
+        "config": {"workload": f"nonlinear thermal J+R assembly, {n}^3 Q2 hex, 3x3x3 Gauss, k=k0(1+T^2)",
+                   "n_dof": g.n_dof, "nnz": g.nnz, "n_elem": g.n_elem, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": kname, "launches_per_step": launches_per_step,
+                     "mean_launch_ms": round(mean_launch_ms, 5), "bytes_per_launch": B_launch,
+                     "B_asm": B, "B_min": B_min},
+        "cpu_baseline": cpu,
+        "phases_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in prof.items()},
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

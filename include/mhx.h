@@ -1,0 +1,254 @@
+/*
+ * mhx.h — C ABI of the MI355X-native assembly + linear-solve engine.
+ *
+ * Plain C types only (no torch, no HIP types): a Trilinos-side integration
+ * (a custom AssemblyManager / LinearAlgebraInterface) or a ctypes/cgo stub can
+ * bind it directly.  Every entry point returns MHX_OK (0) or a negative
+ * status; the message is available from mhx_last_error().  Nothing throws
+ * across this boundary.
+ *
+ * Conventions kept from MrHyDE (reference paths relative to the MrHyDE tree):
+ *   - residual returned is -F(u)           src/managers/assembly/assemblyManager_scatter.hpp:227
+ *   - Jacobian is +dF/du                   src/managers/assembly/assemblyManager_scatter.hpp:261
+ *   - strong Dirichlet rows: residual 0,   src/managers/assembly/assemblyManager_scatter.hpp:208,253
+ *     Jacobian row = e_d (diag 1); the     src/managers/assembly/assemblyManager_constraints.hpp:125-141
+ *     Dirichlet columns are NOT zeroed
+ *   - u carries the Dirichlet values       src/managers/solver/solverManager_util.hpp:60-146
+ *   - J du = res, u <- u + du              src/managers/solver/solverManager_solvers.hpp:523,602
+ */
+#ifndef MHX_H
+#define MHX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define MHX_OK                 0
+#define MHX_ERR_ARG           -1   /* bad argument / inconsistent description     */
+#define MHX_ERR_DEVICE        -2   /* HIP runtime error                            */
+#define MHX_ERR_UNSUPPORTED   -3   /* element/physics combination not built        */
+#define MHX_ERR_NOT_CONVERGED -4   /* Krylov/Newton hit its iteration cap          */
+#define MHX_ERR_NONFINITE     -5   /* NaN/Inf detected in a residual norm          */
+#define MHX_ERR_NOMEM         -6   /* device allocation failed / over budget       */
+#define MHX_ERR_STATE         -7   /* call order (e.g. solve before assembly)      */
+
+/* ---- physics modules (reference registry src/physics/physicsImporter.cpp) */
+#define MHX_PHYS_THERMAL        1  /* src/physics/thermal.cpp                      */
+#define MHX_PHYS_NAVIER_STOKES  2  /* src/physics/navierstokes.cpp                 */
+
+/* ---- box sides, used for Dirichlet side sets ("left", "right", ...) ---- */
+#define MHX_SIDE_XMIN  1u   /* "left"   */
+#define MHX_SIDE_XMAX  2u   /* "right"  */
+#define MHX_SIDE_YMIN  4u   /* "bottom" */
+#define MHX_SIDE_YMAX  8u   /* "top"    */
+#define MHX_SIDE_ZMIN 16u   /* "back"   */
+#define MHX_SIDE_ZMAX 32u   /* "front"  */
+#define MHX_SIDE_ALL  63u   /* "all boundaries" */
+
+#define MHX_MAX_MODES 8
+
+/*
+ * Problem description: the subset of a MrHyDE input deck this path needs.
+ * Mesh: structured box, NX x NY (x NZ) quads/hexes (Mesh: NX/NY/NZ, xmin..).
+ * Discretization: "order" (HGRAD_{QUAD,HEX}_Cn, equispaced nodes) and the
+ * cubature "quadrature" degree (Gauss, n = degree/2 + 1 points per axis),
+ * src/interfaces/discretization/discretizationInterface_basis.hpp:219-350,
+ * discretizationInterface_integration.hpp:91-106.
+ *
+ * Thermal coefficients (src/physics/thermal.cpp:48-65 function names):
+ *   "thermal diffusion" k(T,x) = k0_e * (1 + diff_u2 * T*T)
+ *   "thermal source"    f(x)   = src_e + sum_m amp_m * prod_d s(freq_md * x_d)
+ *                               with s(w x) = sin(w x) for w != 0 and 1 for w == 0.
+ * Arrays are host pointers, copied at create time; NULL means k0 = 1, src = 0.
+ */
+typedef struct mhx_problem_desc {
+  int32_t dim;            /* 2 or 3 */
+  int32_t order;          /* Lagrange order p >= 1 */
+  int32_t quadrature;     /* cubature degree (reference key "quadrature") */
+  int32_t nx, ny, nz;     /* elements per axis (nz ignored in 2D) */
+  double  xmin, xmax, ymin, ymax, zmin, zmax;
+  double  perturb;        /* interior-vertex jitter, fraction of h (0 = affine mesh) */
+  uint64_t perturb_seed;
+  int32_t physics;        /* MHX_PHYS_* */
+  int32_t workset_size;   /* reference "workset size" (CPU restatement only) */
+
+  /* thermal */
+  const double* k0;       /* [n_elem] or NULL */
+  double  diff_u2;
+  const double* src_elem; /* [n_elem] or NULL */
+  int32_t n_modes;
+  double  mode_amp[MHX_MAX_MODES];
+  double  mode_freq[MHX_MAX_MODES][3];
+
+  /* strong Dirichlet data (reference: "Dirichlet conditions", use strong DBCs) */
+  uint32_t dbc_sides;     /* MHX_SIDE_* mask */
+  double  dbc_const;      /* value when dbc_values == NULL */
+  const double* dbc_values; /* [n_dof] nodal values (only Dirichlet entries read) or NULL */
+
+  /* Navier-Stokes (src/physics/navierstokes.cpp) */
+  double  ns_rho;         /* "density"   */
+  double  ns_mu;          /* "viscosity" */
+  int32_t ns_supg, ns_pspg;
+  uint32_t ns_lid_sides;  /* sides carrying the lid velocity ns_lid_u */
+  double  ns_lid_u[3];
+  int32_t reserved[8];
+} mhx_problem_desc;
+
+typedef struct mhx_ctx mhx_ctx;
+
+/* ---- assembly flags ---------------------------------------------------- */
+#define MHX_RES         1   /* compute residual  (assembleRes)               */
+#define MHX_JAC         2   /* compute Jacobian  (assembleJacRes)            */
+#define MHX_ACCUMULATE  4   /* add into res/jac (reference semantics: caller zeroes) */
+#define MHX_DEVICE_PTRS 8   /* pointer arguments are device pointers         */
+
+/* ---- Krylov options (reference Solver keys) ---------------------------- */
+#define MHX_KRYLOV_CG     1   /* "Pseudo Block CG" / SolverManager::PCG        */
+#define MHX_KRYLOV_GMRES  2   /* "Block GMRES" (default)                       */
+#define MHX_PREC_NONE     0
+#define MHX_PREC_JACOBI   1   /* Ifpack2 RELAXATION "Jacobi"                   */
+#define MHX_PREC_BLOCK_JACOBI 2  /* nodal block Jacobi (multi-field)           */
+#define MHX_PREC_FP32     0x100  /* inner Krylov + preconditioner in FP32, FP64 refinement */
+
+typedef struct mhx_krylov_opts {
+  int32_t solver;        /* MHX_KRYLOV_* */
+  int32_t precond;       /* MHX_PREC_* (| MHX_PREC_FP32) */
+  int32_t max_iters;     /* "max linear iters" */
+  int32_t restart;       /* "Num Blocks" (GMRES) */
+  double  tol;           /* "linear TOL": ||b - A x||_2 / ||b||_2 <= tol */
+  int32_t flags;         /* MHX_DEVICE_PTRS */
+  int32_t inner_iters;   /* FP32 inner iterations per refinement sweep (0 = auto) */
+  double  inner_tol;     /* FP32 inner relative tolerance (0 = auto) */
+} mhx_krylov_opts;
+
+typedef struct mhx_newton_opts {
+  int32_t max_iters;       /* "max nonlinear iters" (default 10) */
+  double  nl_tol;          /* "nonlinear TOL" (relative, default 1e-6) */
+  int32_t backtracking;    /* "allow backtracking" (default true) */
+  int32_t flags;           /* MHX_DEVICE_PTRS */
+  mhx_krylov_opts linear;
+} mhx_newton_opts;
+
+typedef struct mhx_solve_stats {
+  int32_t iters;           /* Krylov iterations (sum over Newton steps) */
+  int32_t nl_iters;        /* Newton iterations */
+  double  rel_res;         /* final FP64 true relative residual ||b-Ax||/||b|| */
+  double  res_norm0;       /* first nonlinear residual inf-norm */
+  double  res_norm;        /* last nonlinear residual inf-norm */
+  int32_t refinements;     /* FP64 refinement sweeps (mixed precision) */
+  int32_t status;
+  double  t_assemble_ms, t_solve_ms;
+} mhx_solve_stats;
+
+/* ---- context lifecycle -------------------------------------------------- */
+/* Builds mesh, DOF numbering, colouring, CRS graph and device storage.
+ * Replaces: MeshInterface/DiscretizationInterface::buildDOFManagers
+ * (src/interfaces/discretization/discretizationInterface_dof.hpp:14-241),
+ * AssemblyManager::createGroups (assemblyManager_groups.hpp:14-476),
+ * LinearAlgebraInterface::setupLinearAlgebra (linearAlgebraInterface_construct.hpp:140-265). */
+int  mhx_ctx_create(const mhx_problem_desc* desc, mhx_ctx** out);
+void mhx_ctx_destroy(mhx_ctx* ctx);
+const char* mhx_last_error(const mhx_ctx* ctx);  /* ctx may be NULL */
+
+/* n_dof, nnz, n_elem, number of element colours */
+int  mhx_sizes(const mhx_ctx* ctx, int64_t* n_dof, int64_t* nnz, int64_t* n_elem, int32_t* n_colors);
+
+/* Host copies of the CRS pattern (rowptr[n_dof+1], colidx[nnz], sorted columns).
+ * Replaces: Tpetra graph from insertGlobalIndices + fillComplete
+ * (src/interfaces/linear_algebra/linearAlgebraInterface_construct.hpp:212-265). */
+int  mhx_crs_pattern(const mhx_ctx* ctx, int32_t* rowptr, int32_t* colidx);
+
+/* mask[n_dof]: 1 for strong-Dirichlet DOFs (AssemblyManager::isFixedDOF,
+ * src/managers/assembly/assemblyManager_constraints.hpp:14-95). */
+int  mhx_dirichlet_mask(const mhx_ctx* ctx, int32_t* mask);
+
+/* u[dbc] = g.  Replaces SolverManager::setDirichlet
+ * (src/managers/solver/solverManager_util.hpp:60-146). */
+int  mhx_set_dirichlet(mhx_ctx* ctx, double* u, int32_t flags);
+
+/* Physical coordinates of every DOF node, xyz[n_dof*3] (host). */
+int  mhx_dof_coords(const mhx_ctx* ctx, double* xyz);
+
+/* Residual and/or Jacobian assembly over all elements.
+ *   u[n_dof] in, res[n_dof] = -F(u) out, jac_vals[nnz] in CRS order out.
+ * Replaces AssemblyManager::assembleJacRes / assembleRes
+ * (src/managers/assembly/assemblyManager.hpp:237-255, :330-345;
+ *  assemblyManager_jacres.hpp:121-630, :638-877) including gather, workset
+ * seeding, physics volumeResidual, scatter and dofConstraints. */
+int  mhx_assemble(mhx_ctx* ctx, const double* u, int32_t flags, double* res, double* jac_vals);
+
+/* y = J x with J in the context's CRS pattern (jac_vals == NULL: the
+ * context's Jacobian).  Replaces Tpetra::CrsMatrix::apply as used by
+ * SolverManager::PCG (src/managers/solver/solverManager_solvers.hpp:845,877). */
+int  mhx_spmv(mhx_ctx* ctx, const double* jac_vals, const double* x, double* y, int32_t flags);
+
+/* Solve J x = r.  jac_vals == NULL uses the Jacobian held by the context
+ * (the last internal assembly).  x is used as the initial guess.
+ * Replaces LinearAlgebraInterface::linearSolver
+ * (src/interfaces/linear_algebra/linearAlgebraInterface_solvers.hpp:14-152)
+ * and SolverManager::PCG (src/managers/solver/solverManager_solvers.hpp:812-905). */
+int  mhx_linear_solve(mhx_ctx* ctx, const double* jac_vals, const double* r, double* x,
+                      const mhx_krylov_opts* opts, mhx_solve_stats* stats);
+
+/* Newton on u in place (u must already carry the Dirichlet values, see
+ * mhx_set_dirichlet).  Replaces SolverManager::nonlinearSolver
+ * (src/managers/solver/solverManager_solvers.hpp:290-643); status 0 on
+ * convergence or when the iteration cap is reached, as the reference does. */
+int  mhx_nonlinear_solve(mhx_ctx* ctx, double* u, const mhx_newton_opts* opts, mhx_solve_stats* stats);
+
+/* L2 errors against a manufactured solution T = sum_m amp_m prod_d s(freq_md x_d)
+ * at the volume cubature points: returns sqrt(sum w (T_h - T)^2) and the
+ * gradient norm.  Replaces PostprocessManager::computeError L2/GRAD
+ * (src/managers/postprocess/postprocessManager_error_estimation.hpp:173-330). */
+int  mhx_l2_error(mhx_ctx* ctx, const double* u, int32_t n_modes, const double* amp,
+                  const double* freq /* [n_modes*3] */, int32_t flags,
+                  double* l2, double* l2_grad);
+
+/* ---- device-resident state --------------------------------------------
+ * Context-owned device vectors, so a caller can keep u, res and J in HBM
+ * between calls and pass them back with MHX_DEVICE_PTRS (the role of the
+ * reference's overlapped Tpetra vectors/matrix created once and reused,
+ * src/managers/solver/solverManager_solvers.hpp:322-333). */
+#define MHX_BUF_U    0   /* [n_dof] */
+#define MHX_BUF_RES  1   /* [n_dof] */
+#define MHX_BUF_JAC  2   /* [nnz]   */
+#define MHX_BUF_DU   3   /* [n_dof] */
+#define MHX_BUF_AUX  4   /* [n_dof] */
+int  mhx_state_ptr(mhx_ctx* ctx, int32_t which, double** dev_ptr, int64_t* count);
+int  mhx_state_upload(mhx_ctx* ctx, int32_t which, const double* host, int64_t count);
+int  mhx_state_download(mhx_ctx* ctx, int32_t which, double* host, int64_t count);
+
+/* ---- profiling: per-phase HIP-event timers (reference Teuchos timers) -- */
+int  mhx_profile_enable(mhx_ctx* ctx, int32_t on);
+/* Returns phase count; fills up to cap names (each <= 47 chars + NUL in a
+ * 48-byte slot), accumulated milliseconds and launch counts. */
+int  mhx_profile_read(mhx_ctx* ctx, int32_t cap, char* names48, double* ms, int64_t* counts);
+int  mhx_profile_reset(mhx_ctx* ctx);
+/* Drain the context's HIP stream. */
+int  mhx_synchronize(mhx_ctx* ctx);
+
+/* Run the context's work on the caller's stream (a hipStream_t passed as
+ * void*; NULL = the legacy default stream), so device buffers produced on
+ * that stream are ordered before the engine reads them.  The context's own
+ * stream is drained first. */
+int  mhx_set_stream(mhx_ctx* ctx, void* hip_stream);
+
+/* ---- seeded inputs ------------------------------------------------------
+ * Counter-based generator (splitmix64 of seed ^ index) so every consumer
+ * produces identical fields.  kind 0: uniform [a,b); kind 1: exp(b*z) with
+ * z ~ N(a?0:0,1) — lognormal with log-mean a, log-sd b. */
+#define MHX_RNG_UNIFORM   0
+#define MHX_RNG_LOGNORMAL 1
+#define MHX_RNG_NORMAL    2
+int  mhx_rng_fill(uint64_t seed, int64_t n, int32_t kind, double a, double b, double* out);
+
+/* Library version / build string. */
+const char* mhx_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MHX_H */

@@ -1,0 +1,175 @@
+"""Constants and ctypes structures of the C ABI (include/mhx.h).
+
+Pure data description: importing this module does not load libmhx.so.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+from typing import Optional, Sequence
+
+import numpy as np
+
+
+MHX_OK = 0
+MHX_ERR_ARG = -1
+MHX_ERR_DEVICE = -2
+MHX_ERR_UNSUPPORTED = -3
+MHX_ERR_NOT_CONVERGED = -4
+MHX_ERR_NONFINITE = -5
+MHX_ERR_NOMEM = -6
+MHX_ERR_STATE = -7
+
+MHX_PHYS_THERMAL = 1
+MHX_PHYS_NAVIER_STOKES = 2
+
+SIDE = {"left": 1, "right": 2, "bottom": 4, "top": 8, "back": 16, "front": 32, "all boundaries": 63}
+MHX_SIDE_ALL = 63
+
+MHX_RES, MHX_JAC, MHX_ACCUMULATE, MHX_DEVICE_PTRS = 1, 2, 4, 8
+MHX_KRYLOV_CG, MHX_KRYLOV_GMRES = 1, 2
+MHX_PREC_NONE, MHX_PREC_JACOBI, MHX_PREC_BLOCK_JACOBI, MHX_PREC_FP32 = 0, 1, 2, 0x100
+MHX_MAX_MODES = 8
+MHX_RNG_UNIFORM, MHX_RNG_LOGNORMAL, MHX_RNG_NORMAL = 0, 1, 2
+MHX_BUF_U, MHX_BUF_RES, MHX_BUF_JAC, MHX_BUF_DU, MHX_BUF_AUX = 0, 1, 2, 3, 4
+
+_dp = C.POINTER(C.c_double)
+
+
+class ProblemDesc(C.Structure):
+    _fields_ = [
+        ("dim", C.c_int32), ("order", C.c_int32), ("quadrature", C.c_int32),
+        ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+        ("xmin", C.c_double), ("xmax", C.c_double), ("ymin", C.c_double), ("ymax", C.c_double),
+        ("zmin", C.c_double), ("zmax", C.c_double),
+        ("perturb", C.c_double), ("perturb_seed", C.c_uint64),
+        ("physics", C.c_int32), ("workset_size", C.c_int32),
+        ("k0", _dp), ("diff_u2", C.c_double), ("src_elem", _dp),
+        ("n_modes", C.c_int32),
+        ("mode_amp", C.c_double * MHX_MAX_MODES),
+        ("mode_freq", (C.c_double * 3) * MHX_MAX_MODES),
+        ("dbc_sides", C.c_uint32), ("dbc_const", C.c_double), ("dbc_values", _dp),
+        ("ns_rho", C.c_double), ("ns_mu", C.c_double), ("ns_supg", C.c_int32), ("ns_pspg", C.c_int32),
+        ("ns_lid_sides", C.c_uint32), ("ns_lid_u", C.c_double * 3),
+        ("reserved", C.c_int32 * 8),
+    ]
+
+
+class KrylovOpts(C.Structure):
+    _fields_ = [
+        ("solver", C.c_int32), ("precond", C.c_int32), ("max_iters", C.c_int32), ("restart", C.c_int32),
+        ("tol", C.c_double), ("flags", C.c_int32), ("inner_iters", C.c_int32), ("inner_tol", C.c_double),
+    ]
+
+
+class NewtonOpts(C.Structure):
+    _fields_ = [
+        ("max_iters", C.c_int32), ("nl_tol", C.c_double), ("backtracking", C.c_int32), ("flags", C.c_int32),
+        ("linear", KrylovOpts),
+    ]
+
+
+class SolveStats(C.Structure):
+    _fields_ = [
+        ("iters", C.c_int32), ("nl_iters", C.c_int32), ("rel_res", C.c_double), ("res_norm0", C.c_double),
+        ("res_norm", C.c_double), ("refinements", C.c_int32), ("status", C.c_int32),
+        ("t_assemble_ms", C.c_double), ("t_solve_ms", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+def rng_fill(seed: int, n: int, kind: int = MHX_RNG_UNIFORM, a: float = 0.0, b: float = 1.0) -> np.ndarray:
+    """The seeded counter-based generator of mhx_rng_fill, vectorised in numpy:
+    u_i = top 53 bits of splitmix64(seed ^ (i*phi + c)) / 2^53; normals by
+    Box-Muller from draws 2i and 2i+1."""
+    M = np.uint64(0xFFFFFFFFFFFFFFFF)
+
+    def mix(idx):
+        with np.errstate(over="ignore"):
+            z = np.uint64(seed & 0xFFFFFFFFFFFFFFFF) ^ (idx * np.uint64(0x9E3779B97F4A7C15) + np.uint64(0x632BE59BD9B4E019))
+            z = z + np.uint64(0x9E3779B97F4A7C15)
+            z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+            z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+            z = z ^ (z >> np.uint64(31))
+        return (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+    i = np.arange(int(n), dtype=np.uint64)
+    u1 = mix(np.uint64(2) * i)
+    if kind == MHX_RNG_UNIFORM:
+        return a + (b - a) * u1
+    u2 = mix(np.uint64(2) * i + np.uint64(1))
+    z = np.sqrt(-2.0 * np.log(1.0 - u1)) * np.cos(2.0 * np.pi * u2)
+    return a + b * z if kind == MHX_RNG_NORMAL else np.exp(a + b * z)
+
+
+@dataclass
+class Problem:
+    """The deck subset this path reads (Mesh, Physics: thermal, Discretization,
+    Dirichlet conditions).  `modes` lists (amplitude, (wx, wy, wz)) source terms
+    f = sum amp * prod_d s(w_d x_d) with s(w x) = sin(w x) for w != 0, else 1."""
+    dim: int = 2
+    order: int = 1
+    quadrature: int = 2
+    n: Sequence[int] = (8, 8, 1)
+    lo: Sequence[float] = (0.0, 0.0, 0.0)
+    hi: Sequence[float] = (1.0, 1.0, 1.0)
+    k0: Optional[np.ndarray] = None
+    diff_u2: float = 0.0
+    src_elem: Optional[np.ndarray] = None
+    modes: Sequence = ()
+    dbc_sides: int = MHX_SIDE_ALL
+    dbc_const: float = 0.0
+    dbc_values: Optional[np.ndarray] = None
+    perturb: float = 0.0
+    perturb_seed: int = 0
+    workset_size: int = 100
+    physics: int = MHX_PHYS_THERMAL
+    _keep: list = field(default_factory=list, repr=False)
+
+    @property
+    def n_elem(self) -> int:
+        return int(np.prod(list(self.n)[: self.dim]))
+
+    @property
+    def n_dof(self) -> int:
+        return int(np.prod([self.order * int(x) + 1 for x in list(self.n)[: self.dim]]))
+
+    def desc(self) -> ProblemDesc:
+        d = ProblemDesc()
+        d.dim, d.order, d.quadrature = self.dim, self.order, self.quadrature
+        n = list(self.n) + [1] * (3 - len(self.n))
+        d.nx, d.ny, d.nz = int(n[0]), int(n[1]), int(n[2]) if self.dim == 3 else 1
+        lo = list(self.lo) + [0.0] * (3 - len(self.lo))
+        hi = list(self.hi) + [1.0] * (3 - len(self.hi))
+        d.xmin, d.ymin, d.zmin = map(float, lo[:3])
+        d.xmax, d.ymax, d.zmax = map(float, hi[:3])
+        d.perturb, d.perturb_seed = float(self.perturb), int(self.perturb_seed)
+        d.physics, d.workset_size = self.physics, int(self.workset_size)
+        self._keep = []
+
+        def arr(a, count):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, dtype=np.float64)
+            if a.size != count:
+                raise ValueError(f"array of size {a.size}, expected {count}")
+            self._keep.append(a)
+            return a.ctypes.data_as(C.POINTER(C.c_double))
+
+        d.k0 = arr(self.k0, self.n_elem)
+        d.src_elem = arr(self.src_elem, self.n_elem)
+        d.diff_u2 = float(self.diff_u2)
+        if len(self.modes) > MHX_MAX_MODES:
+            raise ValueError("too many source modes")
+        d.n_modes = len(self.modes)
+        for m, (amp, freq) in enumerate(self.modes):
+            d.mode_amp[m] = float(amp)
+            fr = list(freq) + [0.0] * (3 - len(freq))
+            for a in range(3):
+                d.mode_freq[m][a] = float(fr[a])
+        d.dbc_sides, d.dbc_const = int(self.dbc_sides), float(self.dbc_const)
+        d.dbc_values = arr(self.dbc_values, self.n_dof)
+        return d
+

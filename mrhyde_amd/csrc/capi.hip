@@ -1,0 +1,671 @@
+// capi.hip — the C ABI (include/mhx.h) over the engine: context setup,
+// assembly, linear and Newton solves, error norms, profiling, seeded inputs.
+// Nothing here throws across the boundary; failures become negative status
+// codes with a message in mhx_last_error().
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+
+#include "mhx_internal.hpp"
+
+namespace mhx {
+
+static thread_local std::string g_last_error;
+
+// ---- profiler ----------------------------------------------------------------
+int Profiler::phase_id(const char* name) {
+  for (size_t i = 0; i < phases.size(); ++i)
+    if (phases[i].name == name) return (int)i;
+  phases.push_back(Phase{name, 0.0, 0});
+  return (int)phases.size() - 1;
+}
+hipEvent_t Profiler::get() {
+  if (!pool.empty()) {
+    hipEvent_t e = pool.back();
+    pool.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  MHX_HIP(hipEventCreate(&e));
+  return e;
+}
+void Profiler::begin(const char* name, hipStream_t s, Pending& out) {
+  out.phase = phase_id(name);
+  out.a = get();
+  out.b = get();
+  MHX_HIP(hipEventRecord(out.a, s));
+}
+void Profiler::end(Pending& p, hipStream_t s) {
+  (void)hipEventRecord(p.b, s);
+  pending.push_back(p);
+  if (pending.size() > 4096) drain();
+}
+void Profiler::drain() {
+  for (Pending& p : pending) {
+    (void)hipEventSynchronize(p.b);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, p.a, p.b);
+    phases[p.phase].ms += ms;
+    phases[p.phase].count += 1;
+    pool.push_back(p.a);
+    pool.push_back(p.b);
+  }
+  pending.clear();
+}
+Profiler::~Profiler() {
+  drain();
+  for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+}
+
+// ---- helpers -----------------------------------------------------------------
+static void validate(const mhx_problem_desc* d) {
+  MHX_REQUIRE(d, MHX_ERR_ARG, "null problem description");
+  MHX_REQUIRE(d->dim == 2 || d->dim == 3, MHX_ERR_ARG, "dim must be 2 or 3");
+  MHX_REQUIRE(d->order >= 1 && d->order <= kMaxP1 - 1, MHX_ERR_ARG, "order out of range");
+  MHX_REQUIRE(d->quadrature >= 0 && d->quadrature / 2 + 1 <= kMaxQ1, MHX_ERR_ARG, "quadrature out of range");
+  MHX_REQUIRE(d->nx >= 1 && d->ny >= 1 && (d->dim == 2 || d->nz >= 1), MHX_ERR_ARG, "element counts must be >= 1");
+  MHX_REQUIRE(d->xmax > d->xmin && d->ymax > d->ymin && (d->dim == 2 || d->zmax > d->zmin), MHX_ERR_ARG,
+              "empty box");
+  MHX_REQUIRE(d->n_modes >= 0 && d->n_modes <= MHX_MAX_MODES, MHX_ERR_ARG, "n_modes out of range");
+  MHX_REQUIRE(d->physics == MHX_PHYS_THERMAL, MHX_ERR_UNSUPPORTED, "physics module not built in this library");
+  MHX_REQUIRE(thermal_supported(d->dim, d->order, d->quadrature / 2 + 1), MHX_ERR_UNSUPPORTED,
+              "no assembly kernel for this dim/order/quadrature");
+}
+
+// copy-in / copy-out of caller vectors for host-pointer calls
+struct VecArg {
+  Ctx& c;
+  const double* user;
+  double* dev;
+  bool device;
+  size_t n;
+  VecArg(Ctx& cc, const double* p, size_t count, bool is_dev, DevBuf<double>& stage, bool copy_in)
+      : c(cc), user(p), dev(nullptr), device(is_dev), n(count) {
+    if (device) {
+      dev = const_cast<double*>(p);
+    } else {
+      stage.alloc(std::max<size_t>(n, 1));
+      dev = stage.p;
+      if (copy_in && p) MHX_HIP(hipMemcpyAsync(dev, p, n * sizeof(double), hipMemcpyHostToDevice, c.stream));
+    }
+  }
+  void copy_out(double* out) {
+    if (!device && out) MHX_HIP(hipMemcpyAsync(out, dev, n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  }
+};
+
+// ---- error norms at the cubature points (computeError L2 / GRAD) --------------
+struct ErrArgs {
+  BoxMesh m;
+  Tables1D t;
+  const double* vx;
+  const double* u;
+  int nm;
+  double amp[MHX_MAX_MODES];
+  double freq[MHX_MAX_MODES][3];
+  double* part;  // [2 * gridDim]
+};
+
+__global__ __launch_bounds__(256) void k_l2err(ErrArgs a) {
+  const int dim = a.m.dim, p = a.m.p, p1 = p + 1, nq1 = a.m.nq1;
+  const int nb = dim == 3 ? p1 * p1 * p1 : p1 * p1;
+  const int nq = dim == 3 ? nq1 * nq1 * nq1 : nq1 * nq1;
+  const int nv = 1 << dim;
+  double e0 = 0.0, e1 = 0.0;
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < a.m.ne; e += (long)gridDim.x * blockDim.x) {
+    int ec[3] = {(int)(e % a.m.nel[0]), (int)((e / a.m.nel[0]) % a.m.nel[1]), (int)(e / ((long)a.m.nel[0] * a.m.nel[1]))};
+    double X[8][3];
+    for (int v = 0; v < nv; ++v) {
+      long vid = (long)(ec[0] + (v & 1)) + (long)a.m.nvx[0] * ((long)(ec[1] + ((v >> 1) & 1)) + (long)a.m.nvx[1] * (long)(ec[2] + ((v >> 2) & 1)));
+      for (int d = 0; d < 3; ++d) X[v][d] = a.vx[vid * 3 + d];
+    }
+    for (int q = 0; q < nq; ++q) {
+      int qi[3] = {q % nq1, (q / nq1) % nq1, dim == 3 ? q / (nq1 * nq1) : 0};
+      double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, x[3] = {0, 0, 0};
+      for (int v = 0; v < nv; ++v) {
+        int vl[3] = {v & 1, (v >> 1) & 1, (v >> 2) & 1};
+        double nvv = 1.0;
+        for (int d = 0; d < dim; ++d) nvv *= a.t.N[vl[d]][qi[d]];
+        for (int r = 0; r < dim; ++r) {
+          x[r] += nvv * X[v][r];
+          for (int s = 0; s < dim; ++s) {
+            double dn = 1.0;
+            for (int d = 0; d < dim; ++d) dn *= (d == s) ? a.t.M[vl[d]][qi[d]] : a.t.N[vl[d]][qi[d]];
+            J[r][s] += X[v][r] * dn;
+          }
+        }
+      }
+      double Ji[3][3], det;
+      if (dim == 2) {
+        det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+        Ji[0][0] = J[1][1] / det; Ji[0][1] = -J[0][1] / det; Ji[1][0] = -J[1][0] / det; Ji[1][1] = J[0][0] / det;
+      } else {
+        double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+        double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+        double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+        det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+        Ji[0][0] = c00 / det; Ji[1][0] = c01 / det; Ji[2][0] = c02 / det;
+        Ji[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) / det;
+        Ji[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) / det;
+        Ji[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) / det;
+        Ji[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) / det;
+        Ji[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) / det;
+        Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) / det;
+      }
+      double w = fabs(det);
+      for (int d = 0; d < dim; ++d) w *= a.t.W[qi[d]];
+      double Th = 0.0, gr[3] = {0, 0, 0};
+      for (int j = 0; j < nb; ++j) {
+        int jl[3] = {j % p1, (j / p1) % p1, dim == 3 ? j / (p1 * p1) : 0};
+        long node = (long)(p * ec[0] + jl[0]) + (long)a.m.nn[0] * ((long)(p * ec[1] + jl[1]) + (long)a.m.nn[1] * (long)(p * ec[2] + jl[2]));
+        double uj = a.u[node];
+        double b = 1.0;
+        for (int d = 0; d < dim; ++d) b *= a.t.B[jl[d]][qi[d]];
+        Th += uj * b;
+        for (int c = 0; c < dim; ++c) {
+          double gg = 1.0;
+          for (int d = 0; d < dim; ++d) gg *= (d == c) ? a.t.D[jl[d]][qi[d]] : a.t.B[jl[d]][qi[d]];
+          gr[c] += uj * gg;
+        }
+      }
+      double gp[3] = {0, 0, 0};
+      for (int c = 0; c < dim; ++c)
+        for (int b = 0; b < dim; ++b) gp[c] += Ji[b][c] * gr[b];
+      double Tt = 0.0, gt[3] = {0, 0, 0};
+      for (int mm = 0; mm < a.nm; ++mm) {
+        double s[3], cc[3];
+        for (int d = 0; d < dim; ++d) {
+          double fw = a.freq[mm][d];
+          s[d] = fw != 0.0 ? sin(fw * x[d]) : 1.0;
+          cc[d] = fw != 0.0 ? fw * cos(fw * x[d]) : 0.0;
+        }
+        double t = a.amp[mm];
+        for (int d = 0; d < dim; ++d) t *= s[d];
+        Tt += t;
+        for (int c = 0; c < dim; ++c) {
+          double g = a.amp[mm];
+          for (int d = 0; d < dim; ++d) g *= (d == c) ? cc[d] : s[d];
+          gt[c] += g;
+        }
+      }
+      e0 += (Th - Tt) * (Th - Tt) * w;
+      for (int c = 0; c < dim; ++c) e1 += (gp[c] - gt[c]) * (gp[c] - gt[c]) * w;
+    }
+  }
+  __shared__ double sh[2][256];
+  sh[0][threadIdx.x] = e0;
+  sh[1][threadIdx.x] = e1;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      sh[0][threadIdx.x] += sh[0][threadIdx.x + s];
+      sh[1][threadIdx.x] += sh[1][threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    a.part[2 * blockIdx.x] = sh[0][0];
+    a.part[2 * blockIdx.x + 1] = sh[1][0];
+  }
+}
+
+// ---- Newton (SolverManager::nonlinearSolver, solverManager_solvers.hpp:290-643) --
+static int newton(Ctx& c, double* u, const mhx_newton_opts& o, mhx_solve_stats* st) {
+  const long n = c.mesh.ndof;
+  c.res.alloc(n);
+  c.jac.alloc(c.nnz);
+  c.aux.alloc(n);
+  c.du.alloc(n);
+  DevBuf<double>& du = c.du;
+  MHX_HIP(hipMemsetAsync(du.p, 0, n * sizeof(double), c.stream));
+  double first = 0.0, alpha = 1.0;
+  int it = 0, status = MHX_OK;
+  bool proceed = true;
+  const int maxit = o.max_iters > 0 ? o.max_iters : 10;
+  const double tol = o.nl_tol > 0 ? o.nl_tol : 1e-6;
+  while (proceed) {
+    assemble_thermal(c, u, c.res.p, nullptr, true, false, false);  // assembleRes (ScalarT)
+    const double nrm = norm_inf(c, c.res.p, n);
+    if (!std::isfinite(nrm)) throw Error{MHX_ERR_NONFINITE, "Newton: non-finite residual norm"};
+    double scaled;
+    if (it == 0) {
+      first = nrm;
+      scaled = 1.0;
+    } else {
+      scaled = nrm / first;
+    }
+    if (st) {
+      if (it == 0) st->res_norm0 = nrm;
+      st->res_norm = nrm;
+    }
+    bool solve = true;
+    if (o.backtracking && scaled > 1.1) {
+      solve = false;
+      alpha *= 0.5;
+      axpy(c, n, -alpha, du.p, u);
+    } else if (scaled < tol || nrm < 1.0e-100) {
+      solve = false;
+      proceed = false;
+    }
+    if (solve) {
+      assemble_thermal(c, u, c.aux.p, c.jac.p, true, true, false);  // assembleJacRes (AD)
+      c.jac_valid = true;
+      MHX_HIP(hipMemsetAsync(du.p, 0, n * sizeof(double), c.stream));
+      int s = krylov_solve(c, c.jac.p, c.res.p, du.p, o.linear, st);
+      if (s != MHX_OK) status = s;
+      alpha = 1.0;
+      axpy(c, n, alpha, du.p, u);
+    }
+    ++it;
+    if (it >= maxit) proceed = false;
+  }
+  if (st) st->nl_iters = it;
+  MHX_HIP(hipStreamSynchronize(c.stream));
+  return status;
+}
+
+}  // namespace mhx
+
+using namespace mhx;
+
+#define MHX_GUARD_BEGIN try {
+#define MHX_GUARD_END(ctxp)                          \
+  }                                                  \
+  catch (const Error& e) {                           \
+    g_last_error = e.msg;                            \
+    if (ctxp) (ctxp)->err = e.msg;                   \
+    return e.code;                                   \
+  }                                                  \
+  catch (const std::exception& e) {                  \
+    g_last_error = e.what();                         \
+    if (ctxp) (ctxp)->err = e.what();                \
+    return MHX_ERR_ARG;                              \
+  }
+
+struct mhx_ctx : public Ctx {};
+
+extern "C" {
+
+const char* mhx_version(void) { return "mhx 0.1 (gfx950, fp64 assembly, sum-factorised AD columns)"; }
+
+const char* mhx_last_error(const mhx_ctx* ctx) {
+  if (ctx && !ctx->err.empty()) return ctx->err.c_str();
+  return g_last_error.c_str();
+}
+
+int mhx_ctx_create(const mhx_problem_desc* desc, mhx_ctx** out) {
+  mhx_ctx* c = nullptr;
+  MHX_GUARD_BEGIN
+  MHX_REQUIRE(out, MHX_ERR_ARG, "null output pointer");
+  *out = nullptr;
+  validate(desc);
+  int ndev = 0;
+  MHX_HIP(hipGetDeviceCount(&ndev));
+  MHX_REQUIRE(ndev > 0, MHX_ERR_DEVICE, "no HIP device");
+  c = new mhx_ctx();
+  c->desc = *desc;
+  // arrays in the description are consumed at create time only
+  MHX_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  c->own_stream = true;
+  build_box(*c);
+  c->desc.k0 = nullptr;
+  c->desc.src_elem = nullptr;
+  c->desc.dbc_values = nullptr;
+  *out = c;
+  return MHX_OK;
+  }
+  catch (const Error& e) {
+    g_last_error = e.msg;
+    if (c) {
+      if (c->stream && c->own_stream) (void)hipStreamDestroy(c->stream);
+      delete c;
+    }
+    return e.code;
+  }
+}
+
+void mhx_ctx_destroy(mhx_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipStreamSynchronize(ctx->stream);
+  ctx->prof.drain();
+  hipStream_t s = ctx->stream;
+  bool own = ctx->own_stream;
+  delete ctx;
+  if (s && own) (void)hipStreamDestroy(s);
+}
+
+int mhx_sizes(const mhx_ctx* ctx, int64_t* n_dof, int64_t* nnz, int64_t* n_elem, int32_t* n_colors) {
+  if (!ctx) return MHX_ERR_ARG;
+  if (n_dof) *n_dof = ctx->mesh.ndof;
+  if (nnz) *nnz = ctx->nnz;
+  if (n_elem) *n_elem = ctx->mesh.ne;
+  if (n_colors) *n_colors = 1 << ctx->mesh.dim;
+  return MHX_OK;
+}
+
+int mhx_crs_pattern(const mhx_ctx* cctx, int32_t* rowptr, int32_t* colidx) {
+  mhx_ctx* ctx = const_cast<mhx_ctx*>(cctx);
+  MHX_GUARD_BEGIN
+  MHX_REQUIRE(ctx, MHX_ERR_ARG, "null ctx");
+  if (rowptr) std::copy(ctx->h_rowptr.begin(), ctx->h_rowptr.end(), rowptr);
+  if (colidx) {
+    MHX_HIP(hipMemcpyAsync(colidx, ctx->colidx.p, ctx->nnz * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+    MHX_HIP(hipStreamSynchronize(ctx->stream));
+  }
+  return MHX_OK;
+  MHX_GUARD_END(ctx)
+}
+
+int mhx_dirichlet_mask(const mhx_ctx* ctx, int32_t* mask) {
+  if (!ctx || !mask) return MHX_ERR_ARG;
+  for (long g = 0; g < ctx->mesh.ndof; ++g) mask[g] = ctx->h_fixed[g];
+  return MHX_OK;
+}
+
+int mhx_set_dirichlet(mhx_ctx* ctx, double* u, int32_t flags) {
+  MHX_GUARD_BEGIN
+  MHX_REQUIRE(ctx && u, MHX_ERR_ARG, "null argument");
+  if (flags & MHX_DEVICE_PTRS) {
+    set_dirichlet_dev(*ctx, u);
+    MHX_HIP(hipStreamSynchronize(ctx->stream));
+  } else {
+    for (int32_t g : ctx->h_dbc_list) u[g] = ctx->h_dbc[g];
+  }
+  return MHX_OK;
+  MHX_GUARD_END(ctx)
+}
+
+int mhx_dof_coords(const mhx_ctx* ctx, double* xyz) {
+  if (!ctx || !xyz) return MHX_ERR_ARG;
+  const BoxMesh& m = ctx->mesh;
+  for (long g = 0; g < m.ndof; ++g) {
+    long c[3] = {g % m.nn[0], (g / m.nn[0]) % m.nn[1], g / ((long)m.nn[0] * m.nn[1])};
+    long ec[3] = {0, 0, 0};
+    double xi[3] = {0, 0, 0};
+    for (int a = 0; a < m.dim; ++a) {
+      ec[a] = std::min<long>(c[a] / m.p, m.nel[a] - 1);
+      xi[a] = -1.0 + 2.0 * (double)(c[a] - m.p * ec[a]) / m.p;
+    }
+    double X[3] = {0, 0, 0};
+    for (int v = 0; v < (1 << m.dim); ++v) {
+      long vid = (ec[0] + (v & 1)) + m.nvx[0] * ((ec[1] + ((v >> 1) & 1)) + (long)m.nvx[1] * (ec[2] + ((v >> 2) & 1)));
+      double b = 1.0;
+      for (int a = 0; a < m.dim; ++a) b *= ((v >> a) & 1) ? 0.5 * (1 + xi[a]) : 0.5 * (1 - xi[a]);
+      for (int a = 0; a < 3; ++a) X[a] += b * ctx->h_vx[vid * 3 + a];
+    }
+    for (int a = 0; a < 3; ++a) xyz[g * 3 + a] = X[a];
+  }
+  return MHX_OK;
+}
+
+int mhx_assemble(mhx_ctx* ctx, const double* u, int32_t flags, double* res, double* jac_vals) {
+  MHX_GUARD_BEGIN
+  MHX_REQUIRE(ctx && u, MHX_ERR_ARG, "null argument");
+  const bool want_res = (flags & MHX_RES) != 0, want_jac = (flags & MHX_JAC) != 0;
+  MHX_REQUIRE(want_res || want_jac, MHX_ERR_ARG, "flags must request MHX_RES and/or MHX_JAC");
+  MHX_REQUIRE(!want_res || res, MHX_ERR_ARG, "null residual");
+  MHX_REQUIRE(!want_jac || jac_vals, MHX_ERR_ARG, "null jacobian values");
+  const bool dev = (flags & MHX_DEVICE_PTRS) != 0;
+  const bool acc = (flags & MHX_ACCUMULATE) != 0;
+  const long n = ctx->mesh.ndof;
+  VecArg U(*ctx, u, n, dev, ctx->stage_u, true);
+  VecArg Rv(*ctx, res, want_res ? n : 0, dev, ctx->stage_r, acc);
+  VecArg Jv(*ctx, jac_vals, want_jac ? ctx->nnz : 0, dev, ctx->stage_j, acc);
+  assemble_thermal(*ctx, U.dev, want_res ? Rv.dev : nullptr, want_jac ? Jv.dev : nullptr, want_res, want_jac, acc);
+  if (want_res) Rv.copy_out(res);
+  if (want_jac) Jv.copy_out(jac_vals);
+  if (!dev) MHX_HIP(hipStreamSynchronize(ctx->stream));
+  return MHX_OK;
+  MHX_GUARD_END(ctx)
+}
+
+int mhx_spmv(mhx_ctx* ctx, const double* jac_vals, const double* x, double* y, int32_t flags) {
+  MHX_GUARD_BEGIN
+  MHX_REQUIRE(ctx && x && y, MHX_ERR_ARG, "null argument");
+  const bool dev = (flags & MHX_DEVICE_PTRS) != 0;
+  const long n = ctx->mesh.ndof;
+  const double* A;
+  DevBuf<double> jstage;
+  if (jac_vals) {
+    if (dev) {
+      A = jac_vals;
+    } else {
+      jstage.alloc(ctx->nnz);
+      MHX_HIP(hipMemcpyAsync(jstage.p, jac_vals, ctx->nnz * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+      A = jstage.p;
+    }
+  } else {
+    MHX_REQUIRE(ctx->jac_valid && ctx->jac.p, MHX_ERR_STATE, "no Jacobian held by the context");
+    A = ctx->jac.p;
+  }
+  VecArg X(*ctx, x, n, dev, ctx->stage_a, true);
+  VecArg Y(*ctx, y, n, dev, ctx->stage_b, false);
+  spmv(*ctx, A, X.dev, Y.dev);
+  Y.copy_out(y);
+  MHX_HIP(hipStreamSynchronize(ctx->stream));
+  return MHX_OK;
+  MHX_GUARD_END(ctx)
+}
+
+int mhx_linear_solve(mhx_ctx* ctx, const double* jac_vals, const double* r, double* x, const mhx_krylov_opts* opts,
+                     mhx_solve_stats* stats) {
+  MHX_GUARD_BEGIN
+  MHX_REQUIRE(ctx && r && x && opts, MHX_ERR_ARG, "null argument");
+  const bool dev = (opts->flags & MHX_DEVICE_PTRS) != 0;
+  const long n = ctx->mesh.ndof;
+  const double* A;
+  DevBuf<double> jstage;
+  if (jac_vals) {
+    if (dev) {
+      A = jac_vals;
+    } else {
+      jstage.alloc(ctx->nnz);
+      MHX_HIP(hipMemcpyAsync(jstage.p, jac_vals, ctx->nnz * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+      A = jstage.p;
+    }
+  } else {
+    MHX_REQUIRE(ctx->jac_valid && ctx->jac.p, MHX_ERR_STATE, "no Jacobian held by the context");
+    A = ctx->jac.p;
+  }
+  VecArg R(*ctx, r, n, dev, ctx->stage_a, true);
+  VecArg X(*ctx, x, n, dev, ctx->stage_b, true);
+  if (stats) std::memset(stats, 0, sizeof(*stats));
+  int s = krylov_solve(*ctx, A, R.dev, X.dev, *opts, stats);
+  X.copy_out(x);
+  MHX_HIP(hipStreamSynchronize(ctx->stream));
+  if (stats) stats->status = s;
+  return s;
+  MHX_GUARD_END(ctx)
+}
+
+int mhx_nonlinear_solve(mhx_ctx* ctx, double* u, const mhx_newton_opts* opts, mhx_solve_stats* stats) {
+  MHX_GUARD_BEGIN
+  MHX_REQUIRE(ctx && u && opts, MHX_ERR_ARG, "null argument");
+  const bool dev = (opts->flags & MHX_DEVICE_PTRS) != 0;
+  const long n = ctx->mesh.ndof;
+  VecArg U(*ctx, u, n, dev, ctx->stage_u, true);
+  if (stats) std::memset(stats, 0, sizeof(*stats));
+  int s = newton(*ctx, U.dev, *opts, stats);
+  U.copy_out(u);
+  MHX_HIP(hipStreamSynchronize(ctx->stream));
+  if (stats) stats->status = s;
+  return s;
+  MHX_GUARD_END(ctx)
+}
+
+int mhx_l2_error(mhx_ctx* ctx, const double* u, int32_t n_modes, const double* amp, const double* freq, int32_t flags,
+                 double* l2, double* l2_grad) {
+  MHX_GUARD_BEGIN
+  MHX_REQUIRE(ctx && u && l2 && l2_grad, MHX_ERR_ARG, "null argument");
+  MHX_REQUIRE(n_modes >= 0 && n_modes <= MHX_MAX_MODES, MHX_ERR_ARG, "n_modes out of range");
+  const bool dev = (flags & MHX_DEVICE_PTRS) != 0;
+  VecArg U(*ctx, u, ctx->mesh.ndof, dev, ctx->stage_a, true);
+  ErrArgs a{};
+  a.m = ctx->mesh;
+  a.t = ctx->tab;
+  a.vx = ctx->vx.p;
+  a.u = U.dev;
+  a.nm = n_modes;
+  for (int m = 0; m < n_modes; ++m) {
+    a.amp[m] = amp[m];
+    for (int d = 0; d < 3; ++d) a.freq[m][d] = freq[m * 3 + d];
+  }
+  const int nb = 256;
+  DevBuf<double> part;
+  part.alloc(2 * nb);
+  a.part = part.p;
+  hipLaunchKernelGGL(k_l2err, dim3(nb), dim3(256), 0, ctx->stream, a);
+  MHX_HIP(hipGetLastError());
+  std::vector<double> h(2 * nb);
+  MHX_HIP(hipMemcpyAsync(h.data(), part.p, 2 * nb * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  MHX_HIP(hipStreamSynchronize(ctx->stream));
+  double s0 = 0.0, s1 = 0.0;
+  for (int i = 0; i < nb; ++i) {
+    s0 += h[2 * i];
+    s1 += h[2 * i + 1];
+  }
+  *l2 = std::sqrt(s0);
+  *l2_grad = std::sqrt(s1);
+  return MHX_OK;
+  MHX_GUARD_END(ctx)
+}
+
+int mhx_profile_enable(mhx_ctx* ctx, int32_t on) {
+  if (!ctx) return MHX_ERR_ARG;
+  ctx->prof.on = on != 0;
+  return MHX_OK;
+}
+
+int mhx_profile_read(mhx_ctx* ctx, int32_t cap, char* names48, double* ms, int64_t* counts) {
+  MHX_GUARD_BEGIN
+  MHX_REQUIRE(ctx, MHX_ERR_ARG, "null ctx");
+  MHX_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->prof.drain();
+  const int n = (int)ctx->prof.phases.size();
+  for (int i = 0; i < n && i < cap; ++i) {
+    if (names48) {
+      std::memset(names48 + 48 * i, 0, 48);
+      std::strncpy(names48 + 48 * i, ctx->prof.phases[i].name.c_str(), 47);
+    }
+    if (ms) ms[i] = ctx->prof.phases[i].ms;
+    if (counts) counts[i] = ctx->prof.phases[i].count;
+  }
+  return n;
+  MHX_GUARD_END(ctx)
+}
+
+int mhx_profile_reset(mhx_ctx* ctx) {
+  if (!ctx) return MHX_ERR_ARG;
+  (void)hipStreamSynchronize(ctx->stream);
+  ctx->prof.drain();
+  for (auto& p : ctx->prof.phases) {
+    p.ms = 0.0;
+    p.count = 0;
+  }
+  return MHX_OK;
+}
+
+int mhx_synchronize(mhx_ctx* ctx) {
+  if (!ctx) return MHX_ERR_ARG;
+  hipError_t e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) {
+    ctx->err = hipGetErrorString(e);
+    return MHX_ERR_DEVICE;
+  }
+  return MHX_OK;
+}
+
+static DevBuf<double>& state_buf(mhx_ctx* ctx, int32_t which, size_t& count) {
+  switch (which) {
+    case MHX_BUF_U: count = ctx->mesh.ndof; return ctx->u;
+    case MHX_BUF_RES: count = ctx->mesh.ndof; return ctx->res;
+    case MHX_BUF_JAC: count = ctx->nnz; return ctx->jac;
+    case MHX_BUF_DU: count = ctx->mesh.ndof; return ctx->du;
+    case MHX_BUF_AUX: count = ctx->mesh.ndof; return ctx->aux;
+    default: throw Error{MHX_ERR_ARG, "unknown state buffer"};
+  }
+}
+
+int mhx_state_ptr(mhx_ctx* ctx, int32_t which, double** dev_ptr, int64_t* count) {
+  MHX_GUARD_BEGIN
+  MHX_REQUIRE(ctx && dev_ptr, MHX_ERR_ARG, "null argument");
+  size_t n = 0;
+  DevBuf<double>& b = state_buf(ctx, which, n);
+  if (b.n != n) {
+    b.alloc(n);
+    MHX_HIP(hipMemsetAsync(b.p, 0, n * sizeof(double), ctx->stream));
+    MHX_HIP(hipStreamSynchronize(ctx->stream));
+  }
+  *dev_ptr = b.p;
+  if (count) *count = (int64_t)n;
+  return MHX_OK;
+  MHX_GUARD_END(ctx)
+}
+
+int mhx_state_upload(mhx_ctx* ctx, int32_t which, const double* host, int64_t count) {
+  MHX_GUARD_BEGIN
+  MHX_REQUIRE(ctx && host, MHX_ERR_ARG, "null argument");
+  size_t n = 0;
+  DevBuf<double>& b = state_buf(ctx, which, n);
+  MHX_REQUIRE((size_t)count == n, MHX_ERR_ARG, "count does not match the buffer size");
+  if (b.n != n) b.alloc(n);
+  MHX_HIP(hipMemcpyAsync(b.p, host, n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  MHX_HIP(hipStreamSynchronize(ctx->stream));
+  return MHX_OK;
+  MHX_GUARD_END(ctx)
+}
+
+int mhx_state_download(mhx_ctx* ctx, int32_t which, double* host, int64_t count) {
+  MHX_GUARD_BEGIN
+  MHX_REQUIRE(ctx && host, MHX_ERR_ARG, "null argument");
+  size_t n = 0;
+  DevBuf<double>& b = state_buf(ctx, which, n);
+  MHX_REQUIRE((size_t)count == n, MHX_ERR_ARG, "count does not match the buffer size");
+  MHX_REQUIRE(b.n == n && b.p, MHX_ERR_STATE, "buffer not allocated");
+  MHX_HIP(hipMemcpyAsync(host, b.p, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  MHX_HIP(hipStreamSynchronize(ctx->stream));
+  return MHX_OK;
+  MHX_GUARD_END(ctx)
+}
+
+int mhx_set_stream(mhx_ctx* ctx, void* hip_stream) {
+  if (!ctx) return MHX_ERR_ARG;
+  hipError_t e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) {
+    ctx->err = hipGetErrorString(e);
+    return MHX_ERR_DEVICE;
+  }
+  if (ctx->own_stream && ctx->stream != (hipStream_t)hip_stream) {
+    (void)hipStreamDestroy(ctx->stream);
+    ctx->own_stream = false;
+  }
+  ctx->stream = (hipStream_t)hip_stream;
+  return MHX_OK;
+}
+
+static inline uint64_t splitmix(uint64_t seed, uint64_t idx) {
+  uint64_t z = seed ^ (idx * 0x9E3779B97F4A7C15ull + 0x632BE59BD9B4E019ull);
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+int mhx_rng_fill(uint64_t seed, int64_t n, int32_t kind, double a, double b, double* out) {
+  if (n < 0 || (n && !out)) return MHX_ERR_ARG;
+  for (int64_t i = 0; i < n; ++i) {
+    double u1 = (double)(splitmix(seed, 2 * (uint64_t)i) >> 11) * (1.0 / 9007199254740992.0);
+    if (kind == MHX_RNG_UNIFORM) {
+      out[i] = a + (b - a) * u1;
+    } else {
+      double u2 = (double)(splitmix(seed, 2 * (uint64_t)i + 1) >> 11) * (1.0 / 9007199254740992.0);
+      double z = std::sqrt(-2.0 * std::log(1.0 - u1)) * std::cos(2.0 * M_PI * u2);
+      out[i] = kind == MHX_RNG_NORMAL ? a + b * z : std::exp(a + b * z);
+    }
+  }
+  return MHX_OK;
+}
+
+}  // extern "C"

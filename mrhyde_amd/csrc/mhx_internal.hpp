@@ -1,0 +1,215 @@
+// mhx_internal.hpp — shared declarations of the mhx engine (host + device).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/mhx.h"
+
+namespace mhx {
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+struct Error {
+  int code;
+  std::string msg;
+};
+
+#define MHX_HIP(call)                                                                     \
+  do {                                                                                    \
+    hipError_t e_ = (call);                                                               \
+    if (e_ != hipSuccess)                                                                 \
+      throw ::mhx::Error{MHX_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)}; \
+  } while (0)
+
+#define MHX_REQUIRE(cond, code, text)                     \
+  do {                                                    \
+    if (!(cond)) throw ::mhx::Error{(code), (text)};      \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// 1-D tables passed by value in kernel arguments (kernarg segment, read with
+// scalar loads when the index is wave-uniform).
+// ---------------------------------------------------------------------------
+constexpr int kMaxP1 = 8;   // p + 1
+constexpr int kMaxQ1 = 8;   // Gauss points per axis
+
+struct Tables1D {
+  double B[kMaxP1][kMaxQ1];  // Lagrange value  L_i(xi_q), equispaced nodes
+  double D[kMaxP1][kMaxQ1];  // Lagrange slope  L_i'(xi_q)
+  double N[2][kMaxQ1];       // linear geometry map values
+  double M[2][kMaxQ1];       // linear geometry map slopes
+  double W[kMaxQ1];          // Gauss weights
+};
+
+// Structured box mesh description used by the kernels (all arithmetic,
+// no connectivity tables: element (ex,ey,ez) owns lattice nodes
+// p*e + {0..p} per axis; rows are lexicographic in the node lattice).
+struct BoxMesh {
+  int dim, p, nq1;
+  int nel[3];     // elements per axis
+  int nn[3];      // DOF nodes per axis (p*nel+1)
+  int nvx[3];     // vertices per axis (nel+1)
+  long ne, ndof, nvert;
+};
+
+// Thermal coefficients (compiled form of "thermal diffusion" k0_e*(1+a*T^2)
+// and "thermal source" src_e + sum_m amp_m prod_d s(w_md x_d)).
+struct ThermalCoef {
+  const double* k0;   // [ne] device, or nullptr (== 1)
+  const double* src;  // [ne] device, or nullptr (== 0)
+  double diff_u2;
+  int n_modes;
+  double amp[MHX_MAX_MODES];
+  double freq[MHX_MAX_MODES][3];
+};
+
+struct AsmArgs {
+  BoxMesh m;
+  Tables1D t;
+  ThermalCoef coef;
+  const double* vx;       // [nvert*3] vertex coordinates
+  const double* u;        // [ndof]
+  const int32_t* rowptr;  // [ndof+1]
+  const uint8_t* fixed;   // [ndof]
+  double* res;            // [ndof] or nullptr
+  double* jac;            // [nnz]  or nullptr
+  int color;              // colour bits (cx | cy<<1 | cz<<2)
+  long ncolor;            // elements of this colour
+  int cn[3];              // elements per axis inside this colour
+};
+
+// ---------------------------------------------------------------------------
+// device buffers
+// ---------------------------------------------------------------------------
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void alloc(size_t count) {
+    if (count == n && p) return;
+    release();
+    if (count) {
+      hipError_t e = hipMalloc((void**)&p, count * sizeof(T));
+      if (e != hipSuccess) {
+        p = nullptr;
+        throw Error{MHX_ERR_NOMEM, std::string("hipMalloc failed: ") + hipGetErrorString(e)};
+      }
+    }
+    n = count;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// phase timers (named as the reference's Teuchos timers)
+// ---------------------------------------------------------------------------
+struct Profiler {
+  bool on = false;
+  struct Phase {
+    std::string name;
+    double ms = 0.0;
+    long count = 0;
+  };
+  std::vector<Phase> phases;
+  struct Pending {
+    int phase;
+    hipEvent_t a, b;
+  };
+  std::vector<Pending> pending;
+  std::vector<hipEvent_t> pool;
+  int phase_id(const char* name);
+  hipEvent_t get();
+  void begin(const char* name, hipStream_t s, Pending& out);
+  void end(Pending& p, hipStream_t s);
+  void drain();
+  ~Profiler();
+};
+
+struct ScopedPhase {
+  Profiler& P;
+  hipStream_t s;
+  Profiler::Pending pend{-1, nullptr, nullptr};
+  ScopedPhase(Profiler& p, const char* name, hipStream_t st) : P(p), s(st) {
+    if (P.on) P.begin(name, s, pend);
+  }
+  ~ScopedPhase() {
+    if (P.on && pend.phase >= 0) P.end(pend, s);
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Krylov workspace
+// ---------------------------------------------------------------------------
+struct KrylovWork {
+  DevBuf<double> r, z, p, q, x, w, tmp, minv, scal, V;
+  DevBuf<float> jac32, minv32, r32, x32, V32, w32, tmp32;
+  DevBuf<double> part;
+  DevBuf<unsigned int> counter;
+  int m = 0;
+};
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+struct Ctx {
+  mhx_problem_desc desc{};
+  BoxMesh mesh{};
+  Tables1D tab{};
+  int nb = 0, nq = 0;
+  long nnz = 0;
+  std::vector<int32_t> h_rowptr, h_colidx;
+  std::vector<uint8_t> h_fixed;
+  std::vector<double> h_dbc;   // [ndof] Dirichlet values (0 elsewhere)
+  std::vector<double> h_vx;    // [nvert*3]
+  std::vector<int32_t> h_dbc_list;
+  // device
+  DevBuf<int32_t> rowptr, colidx, dbc_list;
+  DevBuf<uint8_t> fixed;
+  DevBuf<double> vx, k0, src, dbc_vals;
+  DevBuf<double> u, res, jac, du, aux;      // context-owned device state (mhx_state_ptr)
+  DevBuf<double> stage_u, stage_r, stage_j;  // staging for host-pointer calls
+  DevBuf<double> stage_a, stage_b;
+  DevBuf<double> norm_buf;
+  bool jac_valid = false;
+  KrylovWork kw;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  Profiler prof;
+  int spmv_lanes = 16;
+  std::string err;
+};
+
+// mesh / graph (mesh_host.cpp)
+void build_box(Ctx& c);
+void gauss_legendre_desc(int n, double* x, double* w);
+void lagrange_equispaced(int p, double xi, double* L, double* dL);
+
+// assembly (assemble.hip)
+bool thermal_supported(int dim, int p, int nq1);
+void assemble_thermal(Ctx& c, const double* u, double* res, double* jac, bool want_res, bool want_jac,
+                      bool accumulate);
+
+// linear algebra (krylov.hip)
+void spmv(Ctx& c, const double* A, const double* x, double* y);
+double dot(Ctx& c, const double* a, const double* b, long n);
+double norm2(Ctx& c, const double* a, long n);
+double norm_inf(Ctx& c, const double* a, long n);
+int krylov_solve(Ctx& c, const double* A, const double* b, double* x, const mhx_krylov_opts& o,
+                 mhx_solve_stats* st);
+void set_dirichlet_dev(Ctx& c, double* u);
+void axpy(Ctx& c, long n, double a, const double* x, double* y);
+
+}  // namespace mhx

@@ -1,0 +1,277 @@
+"""Host-side mirror of the reference manager interface over the C ABI.
+
+Method names follow the reference so call sites read the same:
+  AssemblyManager::assembleJacRes / assembleRes
+      (src/managers/assembly/assemblyManager.hpp:237-255, :330-345)
+  LinearAlgebraInterface::linearSolver
+      (src/interfaces/linear_algebra/linearAlgebraInterface.hpp:506)
+  SolverManager::nonlinearSolver / setDirichlet
+      (src/managers/solver/solverManager.hpp:139-145, solverManager_util.hpp:60-146)
+  PostprocessManager::computeError
+      (src/managers/postprocess/postprocessManager_error_estimation.hpp:173)
+
+Arrays may be numpy (host) or torch CUDA tensors (device, zero copy).  All
+work runs in libmhx.so on the GPU; a missing library raises at import.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional
+
+import numpy as np
+
+from . import capi
+from .abi import Problem, rng_fill  # noqa: F401
+from .capi import lib
+
+
+class MhxError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"mhx error {code}: {msg}")
+        self.code = code
+
+
+def rng(seed: int, n: int, kind: int = capi.MHX_RNG_UNIFORM, a: float = 0.0, b: float = 1.0) -> np.ndarray:
+    """Seeded counter-based fields (uniform [a,b), lognormal exp(a + b z), normal a + b z)."""
+    out = np.empty(int(n), dtype=np.float64)
+    rc = lib.mhx_rng_fill(C.c_uint64(seed & (2**64 - 1)), int(n), int(kind), float(a), float(b),
+                          out.ctypes.data_as(C.c_void_p))
+    if rc != 0:
+        raise MhxError(rc, "rng_fill")
+    return out
+
+
+def _ptr(a):
+    """(pointer, is_device) for numpy arrays and torch tensors (contiguous, float64/int32)."""
+    if a is None:
+        return None, False
+    if isinstance(a, np.ndarray):
+        if not a.flags["C_CONTIGUOUS"]:
+            raise ValueError("array must be C-contiguous")
+        return a.ctypes.data_as(C.c_void_p), False
+    if hasattr(a, "data_ptr"):
+        if not a.is_contiguous():
+            raise ValueError("tensor must be contiguous")
+        return C.c_void_p(a.data_ptr()), bool(a.is_cuda)
+    raise TypeError(f"unsupported array type {type(a)}")
+
+
+def krylov_opts(solver: str = "gmres", tol: float = 1e-10, max_iters: int = 1000, restart: int = 50,
+                precond: str = "jacobi", device: bool = False, inner_iters: int = 0,
+                inner_tol: float = 0.0) -> capi.KrylovOpts:
+    o = capi.KrylovOpts()
+    o.solver = capi.MHX_KRYLOV_CG if solver.lower() in ("cg", "pcg", "pseudo block cg") else capi.MHX_KRYLOV_GMRES
+    p = precond.lower()
+    o.precond = {"none": 0, "jacobi": 1, "block jacobi": 2}[p.replace("-fp32", "").strip()]
+    if p.endswith("-fp32"):
+        o.precond |= capi.MHX_PREC_FP32
+    o.max_iters, o.restart, o.tol = int(max_iters), int(restart), float(tol)
+    o.flags = capi.MHX_DEVICE_PTRS if device else 0
+    o.inner_iters, o.inner_tol = int(inner_iters), float(inner_tol)
+    return o
+
+
+class Mhx:
+    """One assembly/solve context (mesh, DOFs, colouring, CRS and device state)."""
+
+    def __init__(self, problem: Problem):
+        self.problem = problem
+        self._desc = problem.desc()
+        h = C.c_void_p()
+        rc = lib.mhx_ctx_create(C.byref(self._desc), C.byref(h))
+        if rc != 0:
+            raise MhxError(rc, lib.mhx_last_error(None).decode())
+        self._h = h
+        n, nnz, ne, nc = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int32()
+        lib.mhx_sizes(self._h, C.byref(n), C.byref(nnz), C.byref(ne), C.byref(nc))
+        self.n_dof, self.nnz, self.n_elem, self.n_colors = n.value, nnz.value, ne.value, nc.value
+        self._stream = None
+
+    def _follow_torch_stream(self):
+        """Order the engine's work after torch's current stream (device tensors)."""
+        import torch
+
+        s = torch.cuda.current_stream().cuda_stream
+        if s != self._stream:
+            self._check(lib.mhx_set_stream(self._h, C.c_void_p(s)), "set_stream")
+            self._stream = s
+
+    def set_stream(self, hip_stream: int):
+        self._check(lib.mhx_set_stream(self._h, C.c_void_p(hip_stream)), "set_stream")
+        self._stream = hip_stream
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib.mhx_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def _check(self, rc: int, what: str):
+        if rc < 0:
+            raise MhxError(rc, f"{what}: {lib.mhx_last_error(self._h).decode()}")
+        return rc
+
+    # ---- device-resident state ------------------------------------------------
+    def state_ptr(self, which: int):
+        """Device address (int) and length of a context-owned buffer (MHX_BUF_*)."""
+        p, n = C.c_void_p(), C.c_int64()
+        self._check(lib.mhx_state_ptr(self._h, int(which), C.byref(p), C.byref(n)), "state_ptr")
+        return p.value, n.value
+
+    def state_upload(self, which: int, host: np.ndarray):
+        host = np.ascontiguousarray(host, dtype=np.float64)
+        self._check(lib.mhx_state_upload(self._h, int(which), host.ctypes.data, host.size), "state_upload")
+
+    def state_download(self, which: int) -> np.ndarray:
+        _, n = self.state_ptr(which)
+        out = np.empty(n)
+        self._check(lib.mhx_state_download(self._h, int(which), out.ctypes.data, n), "state_download")
+        return out
+
+    def assemble_resident(self, want_res: bool = True, want_jac: bool = True, accumulate: bool = False):
+        """Assemble from the context's device u into its device res / J (no host traffic)."""
+        pu, _ = self.state_ptr(capi.MHX_BUF_U)
+        pr = self.state_ptr(capi.MHX_BUF_RES)[0] if want_res else None
+        pj = self.state_ptr(capi.MHX_BUF_JAC)[0] if want_jac else None
+        flags = capi.MHX_DEVICE_PTRS | (capi.MHX_RES if want_res else 0) | (capi.MHX_JAC if want_jac else 0)
+        if accumulate:
+            flags |= capi.MHX_ACCUMULATE
+        self._check(lib.mhx_assemble(self._h, C.c_void_p(pu), flags, C.c_void_p(pr) if pr else None,
+                                     C.c_void_p(pj) if pj else None), "assemble_resident")
+
+    # ---- setup products -------------------------------------------------------
+    def crs_pattern(self):
+        rp = np.empty(self.n_dof + 1, dtype=np.int32)
+        ci = np.empty(self.nnz, dtype=np.int32)
+        self._check(lib.mhx_crs_pattern(self._h, rp.ctypes.data, ci.ctypes.data), "crs_pattern")
+        return rp, ci
+
+    def dirichlet_mask(self):
+        m = np.empty(self.n_dof, dtype=np.int32)
+        self._check(lib.mhx_dirichlet_mask(self._h, m.ctypes.data), "dirichlet_mask")
+        return m
+
+    def dof_coords(self):
+        x = np.empty((self.n_dof, 3), dtype=np.float64)
+        self._check(lib.mhx_dof_coords(self._h, x.ctypes.data), "dof_coords")
+        return x
+
+    def setDirichlet(self, u):
+        p, dev = _ptr(u)
+        if dev:
+            self._follow_torch_stream()
+        self._check(lib.mhx_set_dirichlet(self._h, p, capi.MHX_DEVICE_PTRS if dev else 0), "setDirichlet")
+        return u
+
+    # ---- AssemblyManager ------------------------------------------------------
+    def assemble(self, u, res=None, jac=None, accumulate: bool = False):
+        flags = (capi.MHX_RES if res is not None else 0) | (capi.MHX_JAC if jac is not None else 0)
+        if accumulate:
+            flags |= capi.MHX_ACCUMULATE
+        pu, du = _ptr(u)
+        pr, dr = _ptr(res)
+        pj, dj = _ptr(jac)
+        devs = {d for d, a in ((du, u), (dr, res), (dj, jac)) if a is not None}
+        if len(devs) != 1:
+            raise ValueError("u/res/jac must all be host arrays or all device tensors")
+        if devs.pop():
+            flags |= capi.MHX_DEVICE_PTRS
+            self._follow_torch_stream()
+        self._check(lib.mhx_assemble(self._h, pu, flags, pr, pj), "assemble")
+        return res, jac
+
+    def assembleJacRes(self, u):
+        res = np.zeros(self.n_dof)
+        jac = np.zeros(self.nnz)
+        self.assemble(np.ascontiguousarray(u, dtype=np.float64), res, jac)
+        return res, jac
+
+    def assembleRes(self, u):
+        res = np.zeros(self.n_dof)
+        self.assemble(np.ascontiguousarray(u, dtype=np.float64), res, None)
+        return res
+
+    # ---- LinearAlgebraInterface / SolverManager ---------------------------------
+    def apply(self, jac, x, y=None):
+        """y = J x (Tpetra CrsMatrix::apply)."""
+        if y is None:
+            y = np.zeros(self.n_dof) if isinstance(x, np.ndarray) else x.new_zeros(self.n_dof)
+        pj, dj = _ptr(jac)
+        px, dx = _ptr(x)
+        py, dy = _ptr(y)
+        if dx:
+            self._follow_torch_stream()
+        self._check(lib.mhx_spmv(self._h, pj, px, py, capi.MHX_DEVICE_PTRS if dx else 0), "apply")
+        return y
+
+    def linearSolver(self, jac, r, x=None, opts: Optional[capi.KrylovOpts] = None, **kw):
+        opts = opts or krylov_opts(**kw)
+        if x is None:
+            x = np.zeros(self.n_dof)
+        pj, dj = _ptr(jac)
+        pr, dr = _ptr(r)
+        px, dx = _ptr(x)
+        if dr or dx:
+            opts.flags |= capi.MHX_DEVICE_PTRS
+            self._follow_torch_stream()
+        st = capi.SolveStats()
+        rc = lib.mhx_linear_solve(self._h, pj, pr, px, C.byref(opts), C.byref(st))
+        if rc < 0 and rc != capi.MHX_ERR_NOT_CONVERGED:
+            self._check(rc, "linearSolver")
+        return x, st.as_dict()
+
+    def nonlinearSolver(self, u, max_iters: int = 10, nl_tol: float = 1e-6, backtracking: bool = True,
+                        linear: Optional[capi.KrylovOpts] = None, linear_max_iters: int = 1000, **kw):
+        """Newton on u in place ("max nonlinear iters", "nonlinear TOL", "allow
+        backtracking"); `linear` / keyword arguments set the Krylov solve."""
+        o = capi.NewtonOpts()
+        o.max_iters, o.nl_tol, o.backtracking = int(max_iters), float(nl_tol), int(backtracking)
+        o.linear = linear or krylov_opts(max_iters=linear_max_iters, **kw)
+        pu, du = _ptr(u)
+        o.flags = capi.MHX_DEVICE_PTRS if du else 0
+        o.linear.flags = o.flags
+        if du:
+            self._follow_torch_stream()
+        st = capi.SolveStats()
+        rc = lib.mhx_nonlinear_solve(self._h, pu, C.byref(o), C.byref(st))
+        if rc < 0 and rc != capi.MHX_ERR_NOT_CONVERGED:
+            self._check(rc, "nonlinearSolver")
+        return u, st.as_dict()
+
+    def computeError(self, u, modes):
+        amp = np.array([m[0] for m in modes], dtype=np.float64)
+        fr = np.zeros((len(modes), 3))
+        for i, (_, f) in enumerate(modes):
+            fr[i, : len(f)] = f
+        pu, du = _ptr(u)
+        if du:
+            self._follow_torch_stream()
+        l2, l2g = C.c_double(), C.c_double()
+        self._check(lib.mhx_l2_error(self._h, pu, len(modes), amp.ctypes.data, fr.ctypes.data,
+                                     capi.MHX_DEVICE_PTRS if du else 0, C.byref(l2), C.byref(l2g)),
+                    "computeError")
+        return l2.value, l2g.value
+
+    # ---- profiling --------------------------------------------------------------
+    def profile(self, on: bool = True):
+        lib.mhx_profile_enable(self._h, int(on))
+
+    def profile_reset(self):
+        lib.mhx_profile_reset(self._h)
+
+    def profile_read(self) -> dict:
+        cap = 64
+        names = C.create_string_buffer(48 * cap)
+        ms = np.zeros(cap)
+        cnt = np.zeros(cap, dtype=np.int64)
+        n = self._check(lib.mhx_profile_read(self._h, cap, names, ms.ctypes.data, cnt.ctypes.data), "profile_read")
+        out = {}
+        for i in range(min(n, cap)):
+            nm = names.raw[48 * i: 48 * (i + 1)].split(b"\0", 1)[0].decode()
+            out[nm] = (float(ms[i]), int(cnt[i]))
+        return out
+
+    def synchronize(self):
+        self._check(lib.mhx_synchronize(self._h), "synchronize")

@@ -1,0 +1,155 @@
+"""ctypes wrapper of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, and used there only as the checker or the CPU
+baseline.  The product (mrhyde_amd) never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+from mrhyde_amd.abi import KrylovOpts, NewtonOpts, Problem, ProblemDesc, SolveStats
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "liboracle.so"
+
+
+def build() -> Path:
+    subprocess.run(["make", "-s", "-C", str(HERE), "liboracle.so"], check=True)
+    return LIB
+
+
+def _load():
+    if not LIB.exists():
+        build()
+    lib = C.CDLL(str(LIB))
+    vp, i32, i64 = C.c_void_p, C.c_int32, C.c_int64
+    sig = {
+        "orc_last_error": (C.c_char_p, []),
+        "orc_create": (C.c_int, [C.POINTER(ProblemDesc), C.POINTER(vp)]),
+        "orc_destroy": (None, [vp]),
+        "orc_sizes": (C.c_int, [vp] + [C.POINTER(i64)] * 3 + [C.POINTER(i32)] * 3),
+        "orc_pattern": (C.c_int, [vp, vp, vp]),
+        "orc_dirichlet": (C.c_int, [vp, vp, vp]),
+        "orc_set_dirichlet": (C.c_int, [vp, vp]),
+        "orc_lids": (C.c_int, [vp, vp]),
+        "orc_assemble": (C.c_int, [vp, vp, i32, vp, vp]),
+        "orc_linear_solve": (C.c_int, [vp, vp, vp, vp, C.POINTER(KrylovOpts), C.POINTER(SolveStats)]),
+        "orc_newton": (C.c_int, [vp, vp, C.POINTER(NewtonOpts), C.POINTER(SolveStats)]),
+        "orc_l2_error": (C.c_int, [vp, vp, i32, vp, vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+        "orc_elem_basis": (C.c_int, [vp, i64, vp, vp, vp, vp]),
+        "orc_gauss": (C.c_int, [i32, vp, vp]),
+        "orc_dof_coords": (C.c_int, [vp, vp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype, fn.argtypes = res, args
+    return lib
+
+
+lib = _load()
+
+
+def gauss(n: int):
+    x = np.empty(n)
+    w = np.empty(n)
+    lib.orc_gauss(n, x.ctypes.data, w.ctypes.data)
+    return x, w
+
+
+class Oracle:
+    """Reference-algorithm restatement on one problem (same numbering as mhx)."""
+
+    def __init__(self, problem: Problem):
+        self.problem = problem
+        self._desc = problem.desc()
+        h = C.c_void_p()
+        rc = lib.orc_create(C.byref(self._desc), C.byref(h))
+        if rc != 0:
+            raise RuntimeError(lib.orc_last_error().decode())
+        self._h = h
+        n, nnz, ne = C.c_int64(), C.c_int64(), C.c_int64()
+        nb, nq, ad = C.c_int32(), C.c_int32(), C.c_int32()
+        lib.orc_sizes(h, C.byref(n), C.byref(nnz), C.byref(ne), C.byref(nb), C.byref(nq), C.byref(ad))
+        self.n_dof, self.nnz, self.n_elem = n.value, nnz.value, ne.value
+        self.nb, self.nq, self.ad_size = nb.value, nq.value, ad.value
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib.orc_destroy(self._h)
+            self._h = None
+
+    def crs_pattern(self):
+        rp = np.empty(self.n_dof + 1, dtype=np.int32)
+        ci = np.empty(self.nnz, dtype=np.int32)
+        lib.orc_pattern(self._h, rp.ctypes.data, ci.ctypes.data)
+        return rp, ci
+
+    def dirichlet(self):
+        m = np.empty(self.n_dof, dtype=np.int32)
+        v = np.empty(self.n_dof)
+        lib.orc_dirichlet(self._h, m.ctypes.data, v.ctypes.data)
+        return m, v
+
+    def setDirichlet(self, u):
+        lib.orc_set_dirichlet(self._h, u.ctypes.data)
+        return u
+
+    def lids(self):
+        L = np.empty((self.n_elem, self.nb), dtype=np.int32)
+        lib.orc_lids(self._h, L.ctypes.data)
+        return L
+
+    def assembleJacRes(self, u):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        res = np.zeros(self.n_dof)
+        jac = np.zeros(self.nnz)
+        lib.orc_assemble(self._h, u.ctypes.data, 1, res.ctypes.data, jac.ctypes.data)
+        return res, jac
+
+    def assembleRes(self, u):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        res = np.zeros(self.n_dof)
+        lib.orc_assemble(self._h, u.ctypes.data, 0, res.ctypes.data, None)
+        return res
+
+    def linearSolver(self, jac, r, x=None, opts: KrylovOpts = None):
+        x = np.zeros(self.n_dof) if x is None else np.ascontiguousarray(x, dtype=np.float64).copy()
+        st = SolveStats()
+        lib.orc_linear_solve(self._h, np.ascontiguousarray(jac).ctypes.data, np.ascontiguousarray(r).ctypes.data,
+                             x.ctypes.data, C.byref(opts), C.byref(st))
+        return x, st.as_dict()
+
+    def nonlinearSolver(self, u, opts: NewtonOpts):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        st = SolveStats()
+        lib.orc_newton(self._h, u.ctypes.data, C.byref(opts), C.byref(st))
+        return u, st.as_dict()
+
+    def computeError(self, u, modes):
+        amp = np.array([m[0] for m in modes], dtype=np.float64)
+        fr = np.zeros((len(modes), 3))
+        for i, (_, f) in enumerate(modes):
+            fr[i, : len(f)] = f
+        l2, l2g = C.c_double(), C.c_double()
+        lib.orc_l2_error(self._h, np.ascontiguousarray(u).ctypes.data, len(modes), amp.ctypes.data,
+                         fr.ctypes.data, C.byref(l2), C.byref(l2g))
+        return l2.value, l2g.value
+
+    def elem_basis(self, e: int = 0):
+        dim = self.problem.dim
+        phi = np.empty((self.nb, self.nq))
+        grad = np.empty((self.nb, self.nq, dim))
+        wts = np.empty(self.nq)
+        x = np.empty((self.nq, 3))
+        lib.orc_elem_basis(self._h, e, phi.ctypes.data, grad.ctypes.data, wts.ctypes.data, x.ctypes.data)
+        return phi, grad, wts, x
+
+    def dof_coords(self):
+        x = np.empty((self.n_dof, 3))
+        lib.orc_dof_coords(self._h, x.ctypes.data)
+        return x

@@ -1,0 +1,793 @@
+// oracle.cpp — CPU restatement of MrHyDE's element-assembly + Newton/Krylov path.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the correctness oracle: it is
+// imported by tests/, by __graft_entry__.smoke() and by bench.py's
+// cpu_baseline leg, and nowhere else.  The product library (mrhyde_amd/)
+// never links, loads or calls it.
+//
+// It restates the reference algorithm in plain scalar C++ (FP64), following
+// the reference files cited at each function, with a Sacado-SFad-style dual
+// number.  Pinning (what checks each part against something other than a
+// retyped formula) is listed in oracle/PINNING.md and exercised by
+// tests/test_oracle.py:
+//   basis/quadrature/geometry  -> regression/discretization/HGRAD/mrhyde.gold
+//   thermal weak form, DBCs    -> regression/thermal/*/mrhyde.gold L2 errors
+//   dual-number Jacobian       -> central finite differences of the residual
+//   CRS graph                  -> closed-form nnz (3n+1)^d / (8n+1)^d
+//   Krylov                     -> scipy.sparse.linalg.spsolve on the same system
+//
+// The local DOF order inside an element is tensor-lexicographic (x fastest);
+// the reference's Q1 vertex order is a fixed permutation of it (applied by the
+// KAT test).  Global DOFs are numbered lexicographically on the node lattice.
+
+#include "../include/mhx.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+namespace orc {
+
+// ---------------------------------------------------------------------------
+// SFad<N>: value + N derivative slots, forward-mode product/chain rules
+// (Sacado::Fad::SFad semantics; typedefs src/preferences.hpp:79-100,
+//  seeding EvalT(size, index, value) src/tools/workset.cpp:884).
+// ---------------------------------------------------------------------------
+template <int N>
+struct SFad {
+  double v;
+  double d[N];
+  SFad() : v(0.0) { for (int i = 0; i < N; ++i) d[i] = 0.0; }
+  SFad(double x) : v(x) { for (int i = 0; i < N; ++i) d[i] = 0.0; }
+  SFad(int, int idx, double x) : v(x) {
+    for (int i = 0; i < N; ++i) d[i] = 0.0;
+    d[idx] = 1.0;
+  }
+  SFad& operator+=(const SFad& b) { v += b.v; for (int i = 0; i < N; ++i) d[i] += b.d[i]; return *this; }
+  SFad& operator+=(double b) { v += b; return *this; }
+};
+template <int N> inline SFad<N> operator+(const SFad<N>& a, const SFad<N>& b) { SFad<N> r; r.v = a.v + b.v; for (int i = 0; i < N; ++i) r.d[i] = a.d[i] + b.d[i]; return r; }
+template <int N> inline SFad<N> operator-(const SFad<N>& a, const SFad<N>& b) { SFad<N> r; r.v = a.v - b.v; for (int i = 0; i < N; ++i) r.d[i] = a.d[i] - b.d[i]; return r; }
+template <int N> inline SFad<N> operator*(const SFad<N>& a, const SFad<N>& b) { SFad<N> r; r.v = a.v * b.v; for (int i = 0; i < N; ++i) r.d[i] = a.d[i] * b.v + a.v * b.d[i]; return r; }
+template <int N> inline SFad<N> operator/(const SFad<N>& a, const SFad<N>& b) { SFad<N> r; r.v = a.v / b.v; for (int i = 0; i < N; ++i) r.d[i] = (a.d[i] * b.v - a.v * b.d[i]) / (b.v * b.v); return r; }
+template <int N> inline SFad<N> operator+(const SFad<N>& a, double b) { SFad<N> r = a; r.v += b; return r; }
+template <int N> inline SFad<N> operator+(double a, const SFad<N>& b) { SFad<N> r = b; r.v += a; return r; }
+template <int N> inline SFad<N> operator-(const SFad<N>& a, double b) { SFad<N> r = a; r.v -= b; return r; }
+template <int N> inline SFad<N> operator-(double a, const SFad<N>& b) { SFad<N> r; r.v = a - b.v; for (int i = 0; i < N; ++i) r.d[i] = -b.d[i]; return r; }
+template <int N> inline SFad<N> operator*(const SFad<N>& a, double b) { SFad<N> r; r.v = a.v * b; for (int i = 0; i < N; ++i) r.d[i] = a.d[i] * b; return r; }
+template <int N> inline SFad<N> operator*(double a, const SFad<N>& b) { SFad<N> r; r.v = a * b.v; for (int i = 0; i < N; ++i) r.d[i] = a * b.d[i]; return r; }
+template <int N> inline SFad<N> operator-(const SFad<N>& a) { SFad<N> r; r.v = -a.v; for (int i = 0; i < N; ++i) r.d[i] = -a.d[i]; return r; }
+template <int N> inline SFad<N> sqrt(const SFad<N>& a) { SFad<N> r; r.v = std::sqrt(a.v); double s = 0.5 / r.v; for (int i = 0; i < N; ++i) r.d[i] = a.d[i] * s; return r; }
+
+template <int N> inline double val(const SFad<N>& a) { return a.v; }
+template <int N> inline double dx(const SFad<N>& a, int i) { return a.d[i]; }
+inline double val(double a) { return a; }
+inline double dx(double, int) { return 0.0; }
+inline double seeded(double x, int, int) { return x; }
+
+template <class T> struct Seed;
+template <int N> struct Seed<SFad<N>> { static SFad<N> make(int idx, double x) { return SFad<N>(N, idx, x); } };
+template <> struct Seed<double> { static double make(int, double x) { return x; } };
+
+// ---------------------------------------------------------------------------
+// 1-D rules
+// ---------------------------------------------------------------------------
+// Gauss-Legendre points on [-1,1] in DESCENDING order (the order that
+// reproduces regression/discretization/HGRAD/mrhyde.gold), with weights.
+// n = degree/2 + 1 points for a cubature of the given degree
+// (DefaultCubatureFactory, discretizationInterface_integration.hpp:91-106).
+static void gauss_legendre(int n, std::vector<double>& x, std::vector<double>& w) {
+  x.assign(n, 0.0);
+  w.assign(n, 0.0);
+  for (int i = 0; i < n; ++i) {
+    double z = std::cos(M_PI * (i + 0.75) / (n + 0.5));
+    for (int it = 0; it < 100; ++it) {
+      double p0 = 1.0, p1 = 0.0;
+      for (int k = 1; k <= n; ++k) {
+        double p2 = p1;
+        p1 = p0;
+        p0 = ((2.0 * k - 1.0) * z * p1 - (k - 1.0) * p2) / k;
+      }
+      double dp = n * (z * p0 - p1) / (z * z - 1.0);
+      double dz = p0 / dp;
+      z -= dz;
+      if (std::fabs(dz) < 1e-17) {
+        // one more evaluation for the weight
+      }
+      if (std::fabs(dz) < 1e-16) break;
+    }
+    double p0 = 1.0, p1 = 0.0;
+    for (int k = 1; k <= n; ++k) {
+      double p2 = p1;
+      p1 = p0;
+      p0 = ((2.0 * k - 1.0) * z * p1 - (k - 1.0) * p2) / k;
+    }
+    double dp = n * (z * p0 - p1) / (z * z - 1.0);
+    x[i] = z;
+    w[i] = 2.0 / ((1.0 - z * z) * dp * dp);
+  }
+  if (n % 2 == 1) x[n / 2] = 0.0;
+}
+
+// Equispaced Lagrange basis of order p on [-1,1] (HGRAD_*_Cn, EQUISPACED,
+// discretizationInterface_basis.hpp:219-350): value and derivative of L_k at xi.
+static void lagrange1d(int p, double xi, double* L, double* dL) {
+  std::vector<double> nodes(p + 1);
+  for (int k = 0; k <= p; ++k) nodes[k] = -1.0 + 2.0 * k / p;
+  for (int k = 0; k <= p; ++k) {
+    double val = 1.0, der = 0.0;
+    for (int m = 0; m <= p; ++m) {
+      if (m == k) continue;
+      double den = nodes[k] - nodes[m];
+      double prod = 1.0 / den;
+      for (int l = 0; l <= p; ++l) {
+        if (l == k || l == m) continue;
+        prod *= (xi - nodes[l]) / (nodes[k] - nodes[l]);
+      }
+      der += prod;
+      val *= (xi - nodes[m]) / den;
+    }
+    L[k] = val;
+    dL[k] = der;
+  }
+}
+
+static inline double rng_u01(uint64_t seed, uint64_t idx) {
+  uint64_t z = seed ^ (idx * 0x9E3779B97F4A7C15ull + 0x632BE59BD9B4E019ull);
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z = z ^ (z >> 31);
+  return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+}
+
+// ---------------------------------------------------------------------------
+// Problem: structured box mesh, DOFs, reference tables, CRS graph, DBCs.
+// ---------------------------------------------------------------------------
+struct Problem {
+  mhx_problem_desc desc;
+  int dim, p, nq1, nb, nq, nv;       // nb = DOFs/elem, nq = points/elem, nv = vertices/elem
+  int nel[3], nn[3], nvx[3];         // elements, DOF nodes, vertices per axis
+  long ne, ndof, nvert;
+  std::vector<double> k0, src;       // per element
+  std::vector<double> vx;            // vertex coords [nvert*3]
+  std::vector<int> lids;             // [ne*nb]
+  std::vector<double> qx1, qw1;      // 1-D Gauss points/weights
+  std::vector<double> bref, gref;    // [nb*nq], [nb*nq*dim] reference basis / grads
+  std::vector<double> qwref;         // [nq]
+  std::vector<double> vbas, vgrad;   // Q1 geometry map [nv*nq], [nv*nq*dim]
+  std::vector<int> rowptr, colidx;   // CRS
+  std::vector<char> fixed;           // strong-DBC mask
+  std::vector<double> dbcval;        // [ndof]
+  int ad_size;
+
+  void vertex_of_elem(long e, int v, double* X) const;
+  void elem_coords(long e, long* c) const {
+    c[0] = e % nel[0];
+    c[1] = (e / nel[0]) % nel[1];
+    c[2] = dim == 3 ? e / ((long)nel[0] * nel[1]) : 0;
+  }
+};
+
+static std::string g_err;
+
+// Reference-element tables and mesh (buildDOFManagers, getBasis,
+// getQuadrature, SimpleMeshManager-style structured numbering).
+static Problem* build_problem(const mhx_problem_desc* d) {
+  Problem* P = new Problem();
+  P->desc = *d;
+  P->dim = d->dim;
+  P->p = d->order;
+  P->nq1 = d->quadrature / 2 + 1;
+  int dim = P->dim, p = P->p;
+  P->nb = 1; P->nq = 1; P->nv = 1;
+  for (int a = 0; a < dim; ++a) { P->nb *= (p + 1); P->nq *= P->nq1; P->nv *= 2; }
+  P->nel[0] = d->nx; P->nel[1] = d->ny; P->nel[2] = dim == 3 ? d->nz : 1;
+  P->ne = 1; P->ndof = 1; P->nvert = 1;
+  for (int a = 0; a < 3; ++a) {
+    bool act = a < dim;
+    P->nn[a] = act ? p * P->nel[a] + 1 : 1;
+    P->nvx[a] = act ? P->nel[a] + 1 : 1;
+    P->ne *= P->nel[a];
+    P->ndof *= P->nn[a];
+    P->nvert *= P->nvx[a];
+  }
+  // reference AD size selection (assemblyManager_construct.hpp:128-171)
+  int nd = P->nb;
+  P->ad_size = nd <= 2 ? 2 : nd <= 4 ? 4 : nd <= 8 ? 8 : nd <= 16 ? 16 : nd <= 18 ? 18 : nd <= 24 ? 24 : nd <= 32 ? 32 : 64;
+
+  P->k0.assign(P->ne, 1.0);
+  P->src.assign(P->ne, 0.0);
+  if (d->k0) std::copy(d->k0, d->k0 + P->ne, P->k0.begin());
+  if (d->src_elem) std::copy(d->src_elem, d->src_elem + P->ne, P->src.begin());
+
+  // vertices
+  double lo[3] = {d->xmin, d->ymin, d->zmin}, hi[3] = {d->xmax, d->ymax, d->zmax};
+  P->vx.assign(P->nvert * 3, 0.0);
+  for (long v = 0; v < P->nvert; ++v) {
+    long c[3] = {v % P->nvx[0], (v / P->nvx[0]) % P->nvx[1], v / ((long)P->nvx[0] * P->nvx[1])};
+    bool interior = true;
+    for (int a = 0; a < dim; ++a) if (c[a] == 0 || c[a] == P->nvx[a] - 1) interior = false;
+    for (int a = 0; a < dim; ++a) {
+      double h = (hi[a] - lo[a]) / P->nel[a];
+      double x = lo[a] + c[a] * h;
+      if (interior && d->perturb != 0.0)
+        x += d->perturb * h * (2.0 * rng_u01(d->perturb_seed, (uint64_t)v * 3 + a) - 1.0);
+      P->vx[v * 3 + a] = x;
+    }
+  }
+  // connectivity, lexicographic local order
+  P->lids.assign(P->ne * P->nb, 0);
+  for (long e = 0; e < P->ne; ++e) {
+    long c[3];
+    P->elem_coords(e, c);
+    for (int j = 0; j < P->nb; ++j) {
+      int jl[3] = {j % (p + 1), (j / (p + 1)) % (p + 1), dim == 3 ? j / ((p + 1) * (p + 1)) : 0};
+      long g[3];
+      for (int a = 0; a < 3; ++a) g[a] = a < dim ? p * c[a] + jl[a] : 0;
+      P->lids[e * P->nb + j] = (int)(g[0] + P->nn[0] * (g[1] + (long)P->nn[1] * g[2]));
+    }
+  }
+  // 1-D rules and tensor tables
+  gauss_legendre(P->nq1, P->qx1, P->qw1);
+  std::vector<double> L1(P->nq1 * (p + 1)), D1(P->nq1 * (p + 1)), N1(P->nq1 * 2), M1(P->nq1 * 2);
+  for (int q = 0; q < P->nq1; ++q) {
+    lagrange1d(p, P->qx1[q], &L1[q * (p + 1)], &D1[q * (p + 1)]);
+    lagrange1d(1, P->qx1[q], &N1[q * 2], &M1[q * 2]);
+  }
+  int nb = P->nb, nq = P->nq, nq1 = P->nq1;
+  P->bref.assign(nb * nq, 0.0);
+  P->gref.assign(nb * nq * dim, 0.0);
+  P->qwref.assign(nq, 0.0);
+  P->vbas.assign(P->nv * nq, 0.0);
+  P->vgrad.assign(P->nv * nq * dim, 0.0);
+  for (int q = 0; q < nq; ++q) {
+    int qi[3] = {q % nq1, (q / nq1) % nq1, dim == 3 ? q / (nq1 * nq1) : 0};
+    double w = 1.0;
+    for (int a = 0; a < dim; ++a) w *= P->qw1[qi[a]];
+    P->qwref[q] = w;
+    for (int j = 0; j < nb; ++j) {
+      int jl[3] = {j % (p + 1), (j / (p + 1)) % (p + 1), dim == 3 ? j / ((p + 1) * (p + 1)) : 0};
+      double b = 1.0;
+      for (int a = 0; a < dim; ++a) b *= L1[qi[a] * (p + 1) + jl[a]];
+      P->bref[j * nq + q] = b;
+      for (int c = 0; c < dim; ++c) {
+        double g = 1.0;
+        for (int a = 0; a < dim; ++a) g *= (a == c ? D1 : L1)[qi[a] * (p + 1) + jl[a]];
+        P->gref[(j * nq + q) * dim + c] = g;
+      }
+    }
+    for (int v = 0; v < P->nv; ++v) {
+      int vl[3] = {v & 1, (v >> 1) & 1, (v >> 2) & 1};
+      double b = 1.0;
+      for (int a = 0; a < dim; ++a) b *= N1[qi[a] * 2 + vl[a]];
+      P->vbas[v * nq + q] = b;
+      for (int c = 0; c < dim; ++c) {
+        double g = 1.0;
+        for (int a = 0; a < dim; ++a) g *= (a == c ? M1 : N1)[qi[a] * 2 + vl[a]];
+        P->vgrad[(v * nq + q) * dim + c] = g;
+      }
+    }
+  }
+  // CRS graph: union over elements of (gid_i x all gids of element), sorted
+  // (linearAlgebraInterface_construct.hpp:243-265).
+  std::vector<std::vector<int>> rows(P->ndof);
+  for (long e = 0; e < P->ne; ++e) {
+    const int* L = &P->lids[e * nb];
+    for (int i = 0; i < nb; ++i)
+      for (int j = 0; j < nb; ++j) rows[L[i]].push_back(L[j]);
+  }
+  P->rowptr.assign(P->ndof + 1, 0);
+  for (long r = 0; r < P->ndof; ++r) {
+    auto& R = rows[r];
+    std::sort(R.begin(), R.end());
+    R.erase(std::unique(R.begin(), R.end()), R.end());
+    P->rowptr[r + 1] = P->rowptr[r] + (int)R.size();
+  }
+  P->colidx.resize(P->rowptr[P->ndof]);
+  for (long r = 0; r < P->ndof; ++r)
+    std::copy(rows[r].begin(), rows[r].end(), P->colidx.begin() + P->rowptr[r]);
+  // strong Dirichlet DOFs on the requested sides (setDirichletData,
+  // discretizationInterface_dof.hpp:643-736) and their values.
+  P->fixed.assign(P->ndof, 0);
+  P->dbcval.assign(P->ndof, 0.0);
+  for (long g = 0; g < P->ndof; ++g) {
+    long c[3] = {g % P->nn[0], (g / P->nn[0]) % P->nn[1], g / ((long)P->nn[0] * P->nn[1])};
+    bool f = false;
+    for (int a = 0; a < dim; ++a) {
+      if ((d->dbc_sides & (1u << (2 * a))) && c[a] == 0) f = true;
+      if ((d->dbc_sides & (1u << (2 * a + 1))) && c[a] == P->nn[a] - 1) f = true;
+    }
+    if (f) {
+      P->fixed[g] = 1;
+      P->dbcval[g] = d->dbc_values ? d->dbc_values[g] : d->dbc_const;
+    }
+  }
+  return P;
+}
+
+void Problem::vertex_of_elem(long e, int v, double* X) const {
+  long c[3];
+  elem_coords(e, c);
+  long vc[3] = {c[0] + (v & 1), c[1] + ((v >> 1) & 1), dim == 3 ? c[2] + ((v >> 2) & 1) : 0};
+  long vid = vc[0] + nvx[0] * (vc[1] + (long)nvx[1] * vc[2]);
+  for (int a = 0; a < 3; ++a) X[a] = vx[vid * 3 + a];
+}
+
+// Physical integration data for one element: wts = w*|det J|, physical
+// gradients J^{-T} grad_ref, and physical point coordinates
+// (getPhysicalVolumetricBasis / getPhysicalIntegrationData,
+//  discretizationInterface_basis.hpp:382-615, _integration.hpp:218-265).
+struct ElemGeom {
+  std::vector<double> wts, grad, x;  // [nq], [nb*nq*dim], [nq*3]
+};
+
+static void elem_geometry(const Problem& P, long e, ElemGeom& G) {
+  int dim = P.dim, nq = P.nq, nb = P.nb, nv = P.nv;
+  G.wts.assign(nq, 0.0);
+  G.grad.assign(nb * nq * dim, 0.0);
+  G.x.assign(nq * 3, 0.0);
+  std::vector<double> X(nv * 3);
+  for (int v = 0; v < nv; ++v) P.vertex_of_elem(e, v, &X[v * 3]);
+  for (int q = 0; q < nq; ++q) {
+    double J[3][3] = {{0}};
+    for (int v = 0; v < nv; ++v) {
+      for (int a = 0; a < dim; ++a) {
+        G.x[q * 3 + a] += P.vbas[v * nq + q] * X[v * 3 + a];
+        for (int b = 0; b < dim; ++b) J[a][b] += X[v * 3 + a] * P.vgrad[(v * nq + q) * dim + b];
+      }
+    }
+    double Ji[3][3] = {{0}};
+    double det;
+    if (dim == 2) {
+      det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+      Ji[0][0] = J[1][1] / det; Ji[0][1] = -J[0][1] / det;
+      Ji[1][0] = -J[1][0] / det; Ji[1][1] = J[0][0] / det;
+    } else {
+      det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+            J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+            J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+      Ji[0][0] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]) / det;
+      Ji[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) / det;
+      Ji[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) / det;
+      Ji[1][0] = (J[1][2] * J[2][0] - J[1][0] * J[2][2]) / det;
+      Ji[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) / det;
+      Ji[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) / det;
+      Ji[2][0] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) / det;
+      Ji[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) / det;
+      Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) / det;
+    }
+    G.wts[q] = P.qwref[q] * std::fabs(det);
+    for (int j = 0; j < nb; ++j)
+      for (int a = 0; a < dim; ++a) {
+        double s = 0.0;
+        for (int b = 0; b < dim; ++b) s += Ji[b][a] * P.gref[(j * nq + q) * dim + b];
+        G.grad[(j * nq + q) * dim + a] = s;
+      }
+  }
+}
+
+// thermal source f(x) = src_e + sum_m amp_m prod_d s(w_md x_d)
+static double source_at(const Problem& P, long e, const double* x) {
+  double f = P.src[e];
+  for (int m = 0; m < P.desc.n_modes; ++m) {
+    double t = P.desc.mode_amp[m];
+    for (int a = 0; a < P.dim; ++a) {
+      double w = P.desc.mode_freq[m][a];
+      if (w != 0.0) t = t * std::sin(w * x[a]);
+    }
+    f += t;
+  }
+  return f;
+}
+
+static inline long find_slot(const Problem& P, int row, int col) {
+  const int* b = &P.colidx[P.rowptr[row]];
+  const int* en = &P.colidx[P.rowptr[row + 1]];
+  const int* it = std::lower_bound(b, en, col);
+  return (long)(it - &P.colidx[0]);
+}
+
+// assembleJacRes / assembleRes for the thermal module.
+// Structure follows assemblyManager_jacres.hpp:121-630 (sequential loop over
+// worksets of `workset size` elements: gather -> seed -> physics -> scatter),
+// with the field evaluation of workset.cpp:979-1115, the thermal volume
+// residual of thermal.cpp:71-165 (steady, no advection) and the scatter of
+// assemblyManager_scatter.hpp:164-278; then dofConstraints
+// (assemblyManager_constraints.hpp:241-270).
+template <class EvalT>
+static void assemble_thermal(const Problem& P, const double* u, bool want_jac, double* res, double* jv) {
+  int ws = P.desc.workset_size > 0 ? P.desc.workset_size : 100;
+  int nb = P.nb, nq = P.nq, dim = P.dim;
+  std::vector<EvalT> uAD, T, dT;
+  ElemGeom G;
+  for (long e0 = 0; e0 < P.ne; e0 += ws) {
+    long nel = std::min<long>(ws, P.ne - e0);
+    uAD.assign(nel * nb, EvalT(0.0));
+    // performGather + computeSolnSteadySeeded
+    for (long le = 0; le < nel; ++le)
+      for (int j = 0; j < nb; ++j) {
+        double uv = u[P.lids[(e0 + le) * nb + j]];
+        uAD[le * nb + j] = want_jac ? Seed<EvalT>::make(j, uv) : EvalT(uv);
+      }
+    for (long le = 0; le < nel; ++le) {
+      long e = e0 + le;
+      elem_geometry(P, e, G);
+      // evaluateSolutionField: field = u_0 b_0, then += u_dof b_dof
+      T.assign(nq, EvalT(0.0));
+      dT.assign(nq * dim, EvalT(0.0));
+      for (int q = 0; q < nq; ++q) {
+        T[q] = uAD[le * nb] * P.bref[q];
+        for (int a = 0; a < dim; ++a) dT[q * dim + a] = uAD[le * nb] * G.grad[(0 * nq + q) * dim + a];
+        for (int j = 1; j < nb; ++j) {
+          T[q] += uAD[le * nb + j] * P.bref[j * nq + q];
+          for (int a = 0; a < dim; ++a) dT[q * dim + a] += uAD[le * nb + j] * G.grad[(j * nq + q) * dim + a];
+        }
+      }
+      // FunctionManager::evaluate("thermal source" / "thermal diffusion")
+      std::vector<double> source(nq);
+      std::vector<EvalT> diff(nq);
+      for (int q = 0; q < nq; ++q) {
+        source[q] = source_at(P, e, &G.x[q * 3]);
+        EvalT TT = T[q] * T[q];
+        EvalT onep = 1.0 + TT * P.desc.diff_u2;
+        diff[q] = P.k0[e] * (P.desc.diff_u2 != 0.0 ? onep : EvalT(1.0));
+      }
+      // "Thermal volume resid 3D part 1" (steady: rho*cp*dTdt == 0)
+      std::vector<EvalT> r(nb, EvalT(0.0));
+      for (int i = 0; i < nb; ++i) {
+        for (int q = 0; q < nq; ++q) {
+          EvalT zero(0.0);
+          r[i] += (zero - source[q]) * G.wts[q] * P.bref[i * nq + q];
+          for (int a = 0; a < dim; ++a)
+            r[i] += diff[q] * dT[q * dim + a] * G.wts[q] * G.grad[(i * nq + q) * dim + a];
+        }
+      }
+      // scatter: res -= r.val(); J(row, cols) += r.dx(col); skip fixed rows
+      const int* L = &P.lids[e * nb];
+      for (int i = 0; i < nb; ++i) {
+        int row = L[i];
+        if (P.fixed[row]) continue;
+        if (res) res[row] += -val(r[i]);
+        if (want_jac && jv)
+          for (int j = 0; j < nb; ++j) jv[find_slot(P, row, L[j])] += dx(r[i], j);
+      }
+    }
+  }
+  if (want_jac && jv)
+    for (long g = 0; g < P.ndof; ++g)
+      if (P.fixed[g]) jv[find_slot(P, (int)g, (int)g)] = 1.0;
+}
+
+static void assemble(const Problem& P, const double* u, bool want_jac, double* res, double* jv) {
+  if (!want_jac) { assemble_thermal<double>(P, u, false, res, nullptr); return; }
+  switch (P.ad_size) {
+    case 2:  assemble_thermal<SFad<2>>(P, u, true, res, jv); break;
+    case 4:  assemble_thermal<SFad<4>>(P, u, true, res, jv); break;
+    case 8:  assemble_thermal<SFad<8>>(P, u, true, res, jv); break;
+    case 16: assemble_thermal<SFad<16>>(P, u, true, res, jv); break;
+    case 18: assemble_thermal<SFad<18>>(P, u, true, res, jv); break;
+    case 24: assemble_thermal<SFad<24>>(P, u, true, res, jv); break;
+    case 32: assemble_thermal<SFad<32>>(P, u, true, res, jv); break;
+    default: assemble_thermal<SFad<64>>(P, u, true, res, jv); break;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Krylov
+// ---------------------------------------------------------------------------
+static void matvec(const Problem& P, const double* A, const double* x, double* y) {
+  for (long r = 0; r < P.ndof; ++r) {
+    double s = 0.0;
+    for (int k = P.rowptr[r]; k < P.rowptr[r + 1]; ++k) s += A[k] * x[P.colidx[k]];
+    y[r] = s;
+  }
+}
+static double dot(long n, const double* a, const double* b) { double s = 0; for (long i = 0; i < n; ++i) s += a[i] * b[i]; return s; }
+
+// SolverManager::PCG (solverManager_solvers.hpp:812-905): Jacobi PCG,
+// stopping on ||r||/||r0|| <= tol with r0 = b - A x0.
+static int pcg(const Problem& P, const double* A, const double* b, double* x, double tol, int maxit, double* relres) {
+  long n = P.ndof;
+  std::vector<double> M(n), p(n, 0.0), q(n, 0.0), r(n), z(n, 0.0);
+  for (long i = 0; i < n; ++i) M[i] = A[find_slot(P, (int)i, (int)i)];
+  matvec(P, A, x, q.data());
+  for (long i = 0; i < n; ++i) r[i] = b[i] - q[i];
+  double rn = std::sqrt(dot(n, r.data(), r.data()));
+  double r0 = rn;
+  double rho = 1.0, rho1;
+  int it = 0;
+  if (r0 == 0.0) { if (relres) *relres = 0.0; return 0; }
+  while (it < maxit && rn / r0 > tol) {
+    for (long i = 0; i < n; ++i) z[i] = r[i] / M[i];
+    rho1 = rho;
+    rho = dot(n, r.data(), z.data());
+    if (it == 0) p = z;
+    else { double beta = rho / rho1; for (long i = 0; i < n; ++i) p[i] = z[i] + beta * p[i]; }
+    matvec(P, A, p.data(), q.data());
+    double alpha = rho / dot(n, p.data(), q.data());
+    for (long i = 0; i < n; ++i) { x[i] += alpha * p[i]; r[i] -= alpha * q[i]; }
+    rn = std::sqrt(dot(n, r.data(), r.data()));
+    ++it;
+  }
+  if (relres) *relres = rn / r0;
+  return it;
+}
+
+// Restarted GMRES(m), right Jacobi preconditioning, modified Gram-Schmidt,
+// Givens rotations; stops on ||b - A x||/||b|| <= tol (true residual checked
+// at each restart).  Stands in for Belos Block GMRES + Ifpack2 Jacobi
+// (linearAlgebraInterface_solvers.hpp:51-80,100-141), whose iterates are
+// not reproducible outside Trilinos: only the solution is compared.
+static int gmres(const Problem& P, const double* A, const double* b, double* x, int m, double tol, int maxit,
+                 double* relres) {
+  long n = P.ndof;
+  std::vector<double> Minv(n), r(n), w(n), tmp(n);
+  for (long i = 0; i < n; ++i) Minv[i] = 1.0 / A[find_slot(P, (int)i, (int)i)];
+  double bn = std::sqrt(dot(n, b, b));
+  if (bn == 0.0) { std::fill(x, x + n, 0.0); if (relres) *relres = 0.0; return 0; }
+  std::vector<std::vector<double>> V(m + 1, std::vector<double>(n));
+  std::vector<double> H((m + 1) * m), cs(m), sn(m), g(m + 1);
+  int it = 0;
+  double rel = 1.0;
+  while (it < maxit) {
+    matvec(P, A, x, tmp.data());
+    for (long i = 0; i < n; ++i) r[i] = b[i] - tmp[i];
+    double beta = std::sqrt(dot(n, r.data(), r.data()));
+    rel = beta / bn;
+    if (rel <= tol) break;
+    for (long i = 0; i < n; ++i) V[0][i] = r[i] / beta;
+    std::fill(g.begin(), g.end(), 0.0);
+    g[0] = beta;
+    int k = 0;
+    for (; k < m && it < maxit; ++k, ++it) {
+      for (long i = 0; i < n; ++i) tmp[i] = Minv[i] * V[k][i];
+      matvec(P, A, tmp.data(), w.data());
+      for (int j = 0; j <= k; ++j) {
+        double h = dot(n, w.data(), V[j].data());
+        H[j * m + k] = h;
+        for (long i = 0; i < n; ++i) w[i] -= h * V[j][i];
+      }
+      double hn = std::sqrt(dot(n, w.data(), w.data()));
+      H[(k + 1) * m + k] = hn;
+      if (hn != 0.0) for (long i = 0; i < n; ++i) V[k + 1][i] = w[i] / hn;
+      for (int j = 0; j < k; ++j) {
+        double a = H[j * m + k], bb = H[(j + 1) * m + k];
+        H[j * m + k] = cs[j] * a + sn[j] * bb;
+        H[(j + 1) * m + k] = -sn[j] * a + cs[j] * bb;
+      }
+      double a = H[k * m + k], bb = H[(k + 1) * m + k];
+      double den = std::sqrt(a * a + bb * bb);
+      cs[k] = a / den; sn[k] = bb / den;
+      H[k * m + k] = den; H[(k + 1) * m + k] = 0.0;
+      g[k + 1] = -sn[k] * g[k];
+      g[k] = cs[k] * g[k];
+      if (std::fabs(g[k + 1]) / bn <= tol * 0.5) { ++k; ++it; break; }
+    }
+    std::vector<double> y(k);
+    for (int j = k - 1; j >= 0; --j) {
+      double s = g[j];
+      for (int l = j + 1; l < k; ++l) s -= H[j * m + l] * y[l];
+      y[j] = s / H[j * m + j];
+    }
+    std::fill(tmp.begin(), tmp.end(), 0.0);
+    for (int j = 0; j < k; ++j) for (long i = 0; i < n; ++i) tmp[i] += y[j] * V[j][i];
+    for (long i = 0; i < n; ++i) x[i] += Minv[i] * tmp[i];
+  }
+  matvec(P, A, x, tmp.data());
+  for (long i = 0; i < n; ++i) r[i] = b[i] - tmp[i];
+  rel = std::sqrt(dot(n, r.data(), r.data())) / bn;
+  if (relres) *relres = rel;
+  return it;
+}
+
+static int linear_solve(const Problem& P, const double* A, const double* b, double* x, const mhx_krylov_opts* o,
+                        mhx_solve_stats* st) {
+  double rel = 0.0;
+  int it;
+  if (o->solver == MHX_KRYLOV_CG) it = pcg(P, A, b, x, o->tol, o->max_iters, &rel);
+  else it = gmres(P, A, b, x, o->restart > 0 ? o->restart : 50, o->tol, o->max_iters, &rel);
+  if (st) { st->iters += it; st->rel_res = rel; }
+  return it;
+}
+
+// SolverManager::nonlinearSolver (solverManager_solvers.hpp:290-643):
+// residual-only assembly, inf-norm test relative to the first norm,
+// optional backtracking, Jacobian assembly, linear solve, u += du.
+static int newton(const Problem& P, double* u, const mhx_newton_opts* o, mhx_solve_stats* st) {
+  long n = P.ndof;
+  std::vector<double> res(n), du(n, 0.0), jv(P.colidx.size());
+  double first = 0.0, alpha = 1.0;
+  int it = 0;
+  bool proceed = true;
+  while (proceed) {
+    std::fill(res.begin(), res.end(), 0.0);
+    assemble(P, u, false, res.data(), nullptr);
+    double nrm = 0.0;
+    for (long i = 0; i < n; ++i) nrm = std::max(nrm, std::fabs(res[i]));
+    double scaled;
+    if (it == 0) { first = nrm; scaled = 1.0; } else scaled = nrm / first;
+    if (st) { if (it == 0) st->res_norm0 = nrm; st->res_norm = nrm; }
+    bool solve = true;
+    if (o->backtracking && scaled > 1.1) {
+      solve = false;
+      alpha *= 0.5;
+      for (long i = 0; i < n; ++i) u[i] -= alpha * du[i];
+    } else if (scaled < o->nl_tol || nrm < 1e-100) {
+      solve = false;
+      proceed = false;
+    }
+    if (solve) {
+      std::fill(jv.begin(), jv.end(), 0.0);
+      std::vector<double> r2(n, 0.0);
+      assemble(P, u, true, r2.data(), jv.data());
+      std::fill(du.begin(), du.end(), 0.0);
+      linear_solve(P, jv.data(), res.data(), du.data(), &o->linear, st);
+      alpha = 1.0;
+      for (long i = 0; i < n; ++i) u[i] += alpha * du[i];
+    }
+    ++it;
+    if (it >= o->max_iters) proceed = false;
+  }
+  if (st) st->nl_iters = it;
+  return 0;
+}
+
+// L2 / H1-seminorm errors at the volume cubature points
+// (postprocessManager_error_estimation.hpp:173-330; sqrt at print,
+//  postprocessManager_util.hpp:778-794).
+static void l2_error(const Problem& P, const double* u, int nm, const double* amp, const double* freq, double* l2,
+                     double* l2g) {
+  double e0 = 0.0, e1 = 0.0;
+  ElemGeom G;
+  int nb = P.nb, nq = P.nq, dim = P.dim;
+  for (long e = 0; e < P.ne; ++e) {
+    elem_geometry(P, e, G);
+    const int* L = &P.lids[e * nb];
+    for (int q = 0; q < nq; ++q) {
+      double Th = 0.0, gh[3] = {0, 0, 0};
+      for (int j = 0; j < nb; ++j) {
+        Th += u[L[j]] * P.bref[j * nq + q];
+        for (int a = 0; a < dim; ++a) gh[a] += u[L[j]] * G.grad[(j * nq + q) * dim + a];
+      }
+      double Tt = 0.0, gt[3] = {0, 0, 0};
+      const double* x = &G.x[q * 3];
+      for (int m = 0; m < nm; ++m) {
+        double s[3], c[3];
+        for (int a = 0; a < dim; ++a) {
+          double w = freq[m * 3 + a];
+          s[a] = w != 0.0 ? std::sin(w * x[a]) : 1.0;
+          c[a] = w != 0.0 ? w * std::cos(w * x[a]) : 0.0;
+        }
+        double t = amp[m];
+        for (int a = 0; a < dim; ++a) t *= s[a];
+        Tt += t;
+        for (int a = 0; a < dim; ++a) {
+          double g = amp[m];
+          for (int b = 0; b < dim; ++b) g *= (a == b ? c[b] : s[b]);
+          gt[a] += g;
+        }
+      }
+      e0 += (Th - Tt) * (Th - Tt) * G.wts[q];
+      for (int a = 0; a < dim; ++a) e1 += (gh[a] - gt[a]) * (gh[a] - gt[a]) * G.wts[q];
+    }
+  }
+  *l2 = std::sqrt(e0);
+  *l2g = std::sqrt(e1);
+}
+
+}  // namespace orc
+
+// ---------------------------------------------------------------------------
+// C entry points used by the tests through ctypes.
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char* orc_last_error(void) { return orc::g_err.c_str(); }
+
+int orc_create(const mhx_problem_desc* d, void** out) {
+  if (!d || !out || d->dim < 2 || d->dim > 3 || d->order < 1 || d->quadrature < 0 || d->nx < 1 || d->ny < 1 ||
+      (d->dim == 3 && d->nz < 1)) {
+    orc::g_err = "orc_create: bad description";
+    return MHX_ERR_ARG;
+  }
+  if (d->physics != MHX_PHYS_THERMAL) { orc::g_err = "orc_create: only thermal"; return MHX_ERR_UNSUPPORTED; }
+  *out = orc::build_problem(d);
+  return MHX_OK;
+}
+void orc_destroy(void* h) { delete (orc::Problem*)h; }
+
+int orc_sizes(void* h, int64_t* ndof, int64_t* nnz, int64_t* ne, int32_t* nb, int32_t* nq, int32_t* ad) {
+  auto* P = (orc::Problem*)h;
+  *ndof = P->ndof; *nnz = (int64_t)P->colidx.size(); *ne = P->ne; *nb = P->nb; *nq = P->nq; *ad = P->ad_size;
+  return MHX_OK;
+}
+int orc_pattern(void* h, int32_t* rowptr, int32_t* colidx) {
+  auto* P = (orc::Problem*)h;
+  std::copy(P->rowptr.begin(), P->rowptr.end(), rowptr);
+  std::copy(P->colidx.begin(), P->colidx.end(), colidx);
+  return MHX_OK;
+}
+int orc_dirichlet(void* h, int32_t* mask, double* vals) {
+  auto* P = (orc::Problem*)h;
+  for (long g = 0; g < P->ndof; ++g) { if (mask) mask[g] = P->fixed[g]; if (vals) vals[g] = P->dbcval[g]; }
+  return MHX_OK;
+}
+int orc_set_dirichlet(void* h, double* u) {
+  auto* P = (orc::Problem*)h;
+  for (long g = 0; g < P->ndof; ++g) if (P->fixed[g]) u[g] = P->dbcval[g];
+  return MHX_OK;
+}
+int orc_lids(void* h, int32_t* lids) {
+  auto* P = (orc::Problem*)h;
+  std::copy(P->lids.begin(), P->lids.end(), lids);
+  return MHX_OK;
+}
+// res/jv are accumulated into (the caller zeroes them, as
+// solverManager_solvers.hpp:347-354 does).
+int orc_assemble(void* h, const double* u, int32_t want_jac, double* res, double* jv) {
+  orc::assemble(*(orc::Problem*)h, u, want_jac != 0, res, jv);
+  return MHX_OK;
+}
+int orc_linear_solve(void* h, const double* A, const double* b, double* x, const mhx_krylov_opts* o,
+                     mhx_solve_stats* st) {
+  orc::linear_solve(*(orc::Problem*)h, A, b, x, o, st);
+  return MHX_OK;
+}
+int orc_newton(void* h, double* u, const mhx_newton_opts* o, mhx_solve_stats* st) {
+  return orc::newton(*(orc::Problem*)h, u, o, st);
+}
+int orc_l2_error(void* h, const double* u, int32_t nm, const double* amp, const double* freq, double* l2,
+                 double* l2g) {
+  orc::l2_error(*(orc::Problem*)h, u, nm, amp, freq, l2, l2g);
+  return MHX_OK;
+}
+// Physical basis values/gradients of element e at its cubature points, in
+// the oracle's lexicographic local order: phi[nb*nq], grad[nb*nq*dim], wts[nq], x[nq*3].
+int orc_elem_basis(void* h, int64_t e, double* phi, double* grad, double* wts, double* x) {
+  auto* P = (orc::Problem*)h;
+  orc::ElemGeom G;
+  orc::elem_geometry(*P, e, G);
+  if (phi) std::copy(P->bref.begin(), P->bref.end(), phi);
+  if (grad) std::copy(G.grad.begin(), G.grad.end(), grad);
+  if (wts) std::copy(G.wts.begin(), G.wts.end(), wts);
+  if (x) std::copy(G.x.begin(), G.x.end(), x);
+  return MHX_OK;
+}
+int orc_gauss(int32_t n, double* x, double* w) {
+  std::vector<double> X, W;
+  orc::gauss_legendre(n, X, W);
+  std::copy(X.begin(), X.end(), x);
+  std::copy(W.begin(), W.end(), w);
+  return MHX_OK;
+}
+int orc_dof_coords(void* h, double* xyz) {
+  // node (ix,iy,iz) of the lattice mapped through the Q1 map of an element containing it
+  auto* P = (orc::Problem*)h;
+  int p = P->p;
+  for (long g = 0; g < P->ndof; ++g) {
+    long c[3] = {g % P->nn[0], (g / P->nn[0]) % P->nn[1], g / ((long)P->nn[0] * P->nn[1])};
+    long ec[3] = {0, 0, 0};
+    double xi[3] = {0, 0, 0};
+    for (int a = 0; a < P->dim; ++a) {
+      ec[a] = std::min<long>(c[a] / p, P->nel[a] - 1);
+      xi[a] = -1.0 + 2.0 * (double)(c[a] - p * ec[a]) / p;
+    }
+    long e = ec[0] + P->nel[0] * (ec[1] + (long)P->nel[1] * ec[2]);
+    double X[3] = {0, 0, 0};
+    for (int v = 0; v < P->nv; ++v) {
+      double V[3];
+      P->vertex_of_elem(e, v, V);
+      double b = 1.0;
+      for (int a = 0; a < P->dim; ++a) b *= ((v >> a) & 1) ? 0.5 * (1 + xi[a]) : 0.5 * (1 - xi[a]);
+      for (int a = 0; a < 3; ++a) X[a] += b * V[a];
+    }
+    for (int a = 0; a < 3; ++a) xyz[g * 3 + a] = X[a];
+  }
+  return MHX_OK;
+}
+
+}  // extern "C"

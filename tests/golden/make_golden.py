@@ -1,0 +1,89 @@
+"""Extract the reference's own known-answer data into small JSON fixtures.
+
+Reads (from a MrHyDE checkout, default /root/reference):
+  regression/discretization/HGRAD/mrhyde.gold     basis values/gradients at Gauss points
+  regression/thermal/*/mrhyde.gold                L2 error norms of manufactured solutions
+and writes tests/golden/hgrad_kat.json and tests/golden/thermal_gold.json.
+Only numbers (inputs and expected outputs) are stored; the deck parameters of
+each case are transcribed next to its gold values.
+
+Usage: python tests/golden/make_golden.py [/path/to/MrHyDE]
+"""
+from __future__ import annotations
+
+import json
+import re
+import sys
+from pathlib import Path
+
+HERE = Path(__file__).resolve().parent
+PI = "pi"
+
+
+def parse_hgrad(path: Path) -> dict:
+    out = {"quad": {"phi": {}, "grad": {}}, "hex": {"phi": {}, "grad": {}}}
+    cur = None
+    for line in path.read_text().splitlines():
+        m = re.match(r"description = (\w+)", line)
+        if m:
+            desc = m.group(1)
+            cur = ("hex" if "Hex" in desc else "quad", "phi" if desc.startswith("Projection") else "grad")
+            continue
+        m = re.match(r"dof (\d+), point (\d+) grad: \(([^)]*)\)", line)
+        if m and cur:
+            out[cur[0]]["grad"].setdefault(m.group(1), {})[m.group(2)] = [float(v) for v in m.group(3).split(",")]
+            continue
+        m = re.match(r"dof (\d+), point (\d+): (\S+)", line)
+        if m and cur:
+            out[cur[0]]["phi"].setdefault(m.group(1), {})[m.group(2)] = float(m.group(3))
+    res = {}
+    for top in ("quad", "hex"):
+        nd = len(out[top]["phi"])
+        res[top] = {
+            "phi": [[out[top]["phi"][str(d)][str(q)] for q in range(nd)] for d in range(nd)],
+            "grad": [[out[top]["grad"][str(d)][str(q)] for q in range(nd)] for d in range(nd)],
+        }
+    return res
+
+
+def norms(path: Path) -> dict:
+    vals = {}
+    for line in path.read_text().splitlines():
+        m = re.search(r"L2 norm of the error for (\S+) = (\S+)", line)
+        if m and m.group(1) not in vals:
+            vals[m.group(1)] = float(m.group(2))
+        m = re.search(r"L2-face norm of the error for (\S+) = (\S+)", line)
+        if m and ("face " + m.group(1)) not in vals:
+            vals["face " + m.group(1)] = float(m.group(2))
+    return vals
+
+
+# deck transcriptions: (gold directory, mesh/discretisation/source/true solution)
+# source and true solution as sums of amp * prod_d sin(w_d x_d) with w in units of pi
+CASES = {
+    "2D_verification": dict(dim=2, order=1, quadrature=2, n=[40, 40], workset=100,
+                            source=[[8.0, [2, 2]]], true=[[1.0, [2, 2]]], nl_iters=2, nl_tol=1e-7),
+    "2D_LinearSolvers": dict(dim=2, order=1, quadrature=2, n=[8, 8], workset=100,
+                             source=[[8.0, [2, 2]]], true=[[1.0, [2, 2]]], nl_iters=2, nl_tol=1e-7),
+    "3D_verification": dict(dim=3, order=1, quadrature=2, n=[10, 10, 10], workset=100,
+                            source=[[12.0, [2, 2, 2]]], true=[[1.0, [2, 2, 2]]], nl_iters=2, nl_tol=1e-7),
+    "2D_verification_highorder": dict(dim=2, order=4, quadrature=8, n=[10, 10], workset=10,
+                                      source=[[8.0, [2, 2]]], true=[[1.0, [2, 2]]], nl_iters=2, nl_tol=1e-11),
+}
+
+
+def main(ref: Path):
+    kat = parse_hgrad(ref / "regression/discretization/HGRAD/mrhyde.gold")
+    (HERE / "hgrad_kat.json").write_text(json.dumps(kat, indent=1))
+    gold = {}
+    for name, deck in CASES.items():
+        g = norms(ref / "regression/thermal" / name / "mrhyde.gold")
+        gold[name] = dict(deck, gold=g)
+    gold["_note"] = ("amplitudes of 'source' are in units of pi^2 (source = amp*pi^2*prod sin(w*pi*x)); "
+                     "'true' amplitudes are plain; frequencies are in units of pi")
+    (HERE / "thermal_gold.json").write_text(json.dumps(gold, indent=1))
+    print("wrote", HERE / "hgrad_kat.json", HERE / "thermal_gold.json")
+
+
+if __name__ == "__main__":
+    main(Path(sys.argv[1]) if len(sys.argv) > 1 else Path("/root/reference"))
