@@ -69,6 +69,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=96, help="elements per axis")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--colouring", type=int, default=0, help="0 element colouring, 1 block colouring")
+    ap.add_argument("--probe-gib", type=float, default=1.0, help="size of the plain-copy bandwidth probe")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
@@ -82,6 +84,7 @@ def main():
     prob = c2_problem(n)
     t0 = time.perf_counter()
     g = Mhx(prob)
+    g.set_option("assembly.colouring", args.colouring)
     t_setup = time.perf_counter() - t0
     u = rng_fill(7, g.n_dof, 0, 0.0, 1.0)
     g.setDirichlet(u)
@@ -114,6 +117,18 @@ def main():
     achieved = B_launch / (mean_launch_ms * 1e-3) / 1e9
     B_min = 8 * g.nnz + 16 * g.n_dof + 8 * 2 * g.n_elem + 8 * 3 * n_vert
 
+    # achievable-bandwidth reference: plain device copy of known size
+    import ctypes as C
+
+    from mrhyde_amd import capi
+
+    probe = None
+    if args.probe_gib > 0:
+        nd = int(args.probe_gib * 2 ** 30 / 8)
+        ms = C.c_double()
+        if capi.lib.mhx_copy_probe(nd, 5, C.byref(ms)) == 0:
+            probe = round(16.0 * nd / (ms.value * 1e-3) / 1e9, 1)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(n)
@@ -137,7 +152,8 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": kname, "launches_per_step": launches_per_step,
                      "mean_launch_ms": round(mean_launch_ms, 5), "bytes_per_launch": B_launch,
-                     "B_asm": B, "B_min": B_min},
+                     "B_asm": B, "B_min": B_min, "copy_probe_GBps": probe,
+                     "colouring": "block" if args.colouring else "element"},
         "cpu_baseline": cpu,
         "phases_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in prof.items()},
     }

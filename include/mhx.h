@@ -207,6 +207,18 @@ int  mhx_l2_error(mhx_ctx* ctx, const double* u, int32_t n_modes, const double* 
                   const double* freq /* [n_modes*3] */, int32_t flags,
                   double* l2, double* l2_grad);
 
+/* ---- tuning options -----------------------------------------------------
+ * Named integer options of a context (the role of the reference's
+ * "workset size" / "assembly partitioning" keys, scripts/input-files/
+ * input-defaults.yaml:124).  Known names:
+ *   "assembly.colouring"  0 = element colouring (default: one launch per
+ *                         element colour), 1 = block colouring (one launch
+ *                         per block colour, element colours as in-workgroup
+ *                         sub-phases).
+ * Returns MHX_ERR_ARG for an unknown name or value. */
+int  mhx_set_option(mhx_ctx* ctx, const char* name, int64_t value);
+int  mhx_get_option(const mhx_ctx* ctx, const char* name, int64_t* value);
+
 /* ---- device-resident state --------------------------------------------
  * Context-owned device vectors, so a caller can keep u, res and J in HBM
  * between calls and pass them back with MHX_DEVICE_PTRS (the role of the
@@ -244,6 +256,12 @@ int  mhx_set_stream(mhx_ctx* ctx, void* hip_stream);
 #define MHX_RNG_LOGNORMAL 1
 #define MHX_RNG_NORMAL    2
 int  mhx_rng_fill(uint64_t seed, int64_t n, int32_t kind, double a, double b, double* out);
+
+/* Plain device copy y = x of n_doubles FP64 values, repeated reps times on a
+ * private stream; returns the mean milliseconds per copy.  A calibration
+ * kernel of known traffic (8 B read + 8 B written per value) for HBM
+ * counters and the achievable-bandwidth figure; not part of the reference. */
+int  mhx_copy_probe(int64_t n_doubles, int32_t reps, double* ms_per_copy);
 
 /* Library version / build string. */
 const char* mhx_version(void);

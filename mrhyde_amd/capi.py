@@ -39,6 +39,9 @@ def _load() -> C.CDLL:
         "mhx_profile_reset": (C.c_int, [vp]),
         "mhx_synchronize": (C.c_int, [vp]),
         "mhx_set_stream": (C.c_int, [vp, vp]),
+        "mhx_copy_probe": (C.c_int, [i64, i32, C.POINTER(C.c_double)]),
+        "mhx_set_option": (C.c_int, [vp, C.c_char_p, i64]),
+        "mhx_get_option": (C.c_int, [vp, C.c_char_p, C.POINTER(i64)]),
         "mhx_state_ptr": (C.c_int, [vp, i32, C.POINTER(C.c_void_p), C.POINTER(i64)]),
         "mhx_state_upload": (C.c_int, [vp, i32, vp, i64]),
         "mhx_state_download": (C.c_int, [vp, i32, vp, i64]),
@@ -58,5 +61,5 @@ EXPORTED = [
     "mhx_ctx_create", "mhx_ctx_destroy", "mhx_last_error", "mhx_sizes", "mhx_crs_pattern",
     "mhx_dirichlet_mask", "mhx_set_dirichlet", "mhx_dof_coords", "mhx_assemble", "mhx_spmv", "mhx_linear_solve",
     "mhx_nonlinear_solve", "mhx_l2_error", "mhx_profile_enable", "mhx_profile_read", "mhx_profile_reset",
-    "mhx_synchronize", "mhx_set_stream", "mhx_state_ptr", "mhx_state_upload", "mhx_state_download", "mhx_rng_fill", "mhx_version",
+    "mhx_synchronize", "mhx_set_stream", "mhx_set_option", "mhx_get_option", "mhx_copy_probe","mhx_state_ptr", "mhx_state_upload", "mhx_state_download", "mhx_rng_fill", "mhx_version",
 ]

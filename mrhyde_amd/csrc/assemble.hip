@@ -46,6 +46,95 @@ constexpr int pow2ceil(int x) {
   return r;
 }
 
+// ---------------------------------------------------------------------------
+// 1-D reference tables as compile-time constants per (order, points): Gauss-
+// Legendre points (descending) and weights, equispaced Lagrange values and
+// slopes, and the linear geometry map.  Uniform-index reads become scalar
+// loads of .rodata that the compiler can re-issue instead of keeping the whole
+// table live in registers.
+// ---------------------------------------------------------------------------
+struct Tab1 {
+  double X[kMaxQ1], W[kMaxQ1];
+  double B[kMaxP1][kMaxQ1], D[kMaxP1][kMaxQ1];
+  double N[2][kMaxQ1], M[2][kMaxQ1];
+};
+
+constexpr double cx_cos(double x) {  // Taylor series, |x| <= pi
+  double term = 1.0, sum = 1.0;
+  for (int k = 1; k < 40; ++k) {
+    term *= -x * x / ((2.0 * k - 1.0) * (2.0 * k));
+    sum += term;
+  }
+  return sum;
+}
+
+constexpr void cx_lagrange(int p, double xi, double* L, double* dL) {
+  double nodes[kMaxP1] = {};
+  for (int k = 0; k <= p; ++k) nodes[k] = -1.0 + 2.0 * k / p;
+  for (int k = 0; k <= p; ++k) {
+    double v = 1.0, d = 0.0;
+    for (int m = 0; m <= p; ++m) {
+      if (m == k) continue;
+      const double den = nodes[k] - nodes[m];
+      double prod = 1.0 / den;
+      for (int l = 0; l <= p; ++l) {
+        if (l == k || l == m) continue;
+        prod *= (xi - nodes[l]) / (nodes[k] - nodes[l]);
+      }
+      d += prod;
+      v *= (xi - nodes[m]) / den;
+    }
+    L[k] = v;
+    dL[k] = d;
+  }
+}
+
+constexpr Tab1 make_tab1(int p, int n) {
+  Tab1 t{};
+  for (int i = 0; i < n; ++i) {
+    double z = cx_cos(3.14159265358979323846 * (i + 0.75) / (n + 0.5));
+    double dp = 1.0;
+    for (int it = 0; it < 60; ++it) {
+      double p0 = 1.0, p1 = 0.0;
+      for (int k = 1; k <= n; ++k) {
+        const double p2 = p1;
+        p1 = p0;
+        p0 = ((2.0 * k - 1.0) * z * p1 - (k - 1.0) * p2) / k;
+      }
+      dp = n * (z * p0 - p1) / (z * z - 1.0);
+      z -= p0 / dp;
+    }
+    double p0 = 1.0, p1 = 0.0;
+    for (int k = 1; k <= n; ++k) {
+      const double p2 = p1;
+      p1 = p0;
+      p0 = ((2.0 * k - 1.0) * z * p1 - (k - 1.0) * p2) / k;
+    }
+    dp = n * (z * p0 - p1) / (z * z - 1.0);
+    t.X[i] = (n % 2 == 1 && i == n / 2) ? 0.0 : z;
+    t.W[i] = 2.0 / ((1.0 - z * z) * dp * dp);
+  }
+  for (int q = 0; q < n; ++q) {
+    double L[kMaxP1] = {}, dL[kMaxP1] = {}, L1[kMaxP1] = {}, dL1[kMaxP1] = {};
+    cx_lagrange(p, t.X[q], L, dL);
+    cx_lagrange(1, t.X[q], L1, dL1);
+    for (int i = 0; i <= p; ++i) {
+      t.B[i][q] = L[i];
+      t.D[i][q] = dL[i];
+    }
+    for (int i = 0; i < 2; ++i) {
+      t.N[i][q] = L1[i];
+      t.M[i][q] = dL1[i];
+    }
+  }
+  return t;
+}
+
+template <int P, int NQ1>
+struct T1 {
+  static constexpr Tab1 v = make_tab1(P, NQ1);
+};
+
 template <int DIM, int P, int NQ1>
 struct Cfg {
   static constexpr int P1 = P + 1;
@@ -138,64 +227,62 @@ __device__ __forceinline__ double thermal_source(const ThermalCoef& cf, long e, 
 // ---------------------------------------------------------------------------
 // sum-factorised application of all test functions:
 //   R[i] = sum_q sum_c f_c(q) ghat_i[c](q),  ghat = (d/dxi_0 .. d/dxi_{d-1}, value)
-// f is produced point by point by the callable `fq(q, f)`.
+// f is produced point by point by the callable `fq(qx, qy, qz, f)`.  The
+// outermost point axis is streamed: each of its planes is contracted over the
+// inner axes and folded into R at once, so only one plane of partial sums is
+// live.
 // ---------------------------------------------------------------------------
 template <int DIM, int P, int NQ1, class FQ>
-__device__ __forceinline__ void test_apply(const Tables1D& t, FQ&& fq, double* R) {
+__device__ __forceinline__ void test_apply(const Tab1& t, FQ&& fq, double* R) {
   constexpr int P1 = P + 1;
+  constexpr int NB = DIM == 2 ? P1 * P1 : P1 * P1 * P1;
+#pragma unroll
+  for (int i = 0; i < NB; ++i) R[i] = 0.0;
   if constexpr (DIM == 2) {
-    double A0[P1][NQ1], A1[P1][NQ1];
 #pragma unroll
-    for (int i = 0; i < P1; ++i)
+    for (int qy = 0; qy < NQ1; ++qy) {
+      double A0[P1], A1[P1];
 #pragma unroll
-      for (int q = 0; q < NQ1; ++q) A0[i][q] = A1[i][q] = 0.0;
-#pragma unroll
-    for (int qy = 0; qy < NQ1; ++qy)
+      for (int i = 0; i < P1; ++i) A0[i] = A1[i] = 0.0;
 #pragma unroll
       for (int qx = 0; qx < NQ1; ++qx) {
         double f[3];
-        fq(qx + NQ1 * qy, f);
+        fq(qx, qy, 0, f);
 #pragma unroll
         for (int ix = 0; ix < P1; ++ix) {
-          A0[ix][qy] = fma(t.D[ix][qx], f[0], fma(t.B[ix][qx], f[2], A0[ix][qy]));
-          A1[ix][qy] = fma(t.B[ix][qx], f[1], A1[ix][qy]);
+          A0[ix] = fma(t.D[ix][qx], f[0], fma(t.B[ix][qx], f[2], A0[ix]));
+          A1[ix] = fma(t.B[ix][qx], f[1], A1[ix]);
         }
       }
 #pragma unroll
-    for (int iy = 0; iy < P1; ++iy)
+      for (int iy = 0; iy < P1; ++iy)
 #pragma unroll
-      for (int ix = 0; ix < P1; ++ix) {
-        double s = 0.0;
-#pragma unroll
-        for (int qy = 0; qy < NQ1; ++qy) s = fma(t.B[iy][qy], A0[ix][qy], fma(t.D[iy][qy], A1[ix][qy], s));
-        R[ix + P1 * iy] = s;
-      }
+        for (int ix = 0; ix < P1; ++ix)
+          R[ix + P1 * iy] = fma(t.B[iy][qy], A0[ix], fma(t.D[iy][qy], A1[ix], R[ix + P1 * iy]));
+    }
   } else {
-    double A0[P1][NQ1][NQ1], A1[P1][NQ1][NQ1], A2[P1][NQ1][NQ1];
 #pragma unroll
-    for (int i = 0; i < P1; ++i)
+    for (int qz = 0; qz < NQ1; ++qz) {
+      // contract x over the plane qz: A0 (x-derivative and value parts), A1 (y), A2 (z)
+      double A0[P1][NQ1], A1[P1][NQ1], A2[P1][NQ1];
 #pragma unroll
-      for (int a = 0; a < NQ1; ++a)
+      for (int i = 0; i < P1; ++i)
 #pragma unroll
-        for (int b = 0; b < NQ1; ++b) A0[i][a][b] = A1[i][a][b] = A2[i][a][b] = 0.0;
-#pragma unroll
-    for (int qz = 0; qz < NQ1; ++qz)
+        for (int b = 0; b < NQ1; ++b) A0[i][b] = A1[i][b] = A2[i][b] = 0.0;
 #pragma unroll
       for (int qy = 0; qy < NQ1; ++qy)
 #pragma unroll
         for (int qx = 0; qx < NQ1; ++qx) {
           double f[4];
-          fq(qx + NQ1 * (qy + NQ1 * qz), f);
+          fq(qx, qy, qz, f);
 #pragma unroll
           for (int ix = 0; ix < P1; ++ix) {
-            A0[ix][qy][qz] = fma(t.D[ix][qx], f[0], fma(t.B[ix][qx], f[3], A0[ix][qy][qz]));
-            A1[ix][qy][qz] = fma(t.B[ix][qx], f[1], A1[ix][qy][qz]);
-            A2[ix][qy][qz] = fma(t.B[ix][qx], f[2], A2[ix][qy][qz]);
+            A0[ix][qy] = fma(t.D[ix][qx], f[0], fma(t.B[ix][qx], f[3], A0[ix][qy]));
+            A1[ix][qy] = fma(t.B[ix][qx], f[1], A1[ix][qy]);
+            A2[ix][qy] = fma(t.B[ix][qx], f[2], A2[ix][qy]);
           }
         }
-    double S[P1][P1][NQ1], Z[P1][P1][NQ1];
-#pragma unroll
-    for (int qz = 0; qz < NQ1; ++qz)
+      // contract y, then fold the plane into R through the z tables
 #pragma unroll
       for (int iy = 0; iy < P1; ++iy)
 #pragma unroll
@@ -203,23 +290,16 @@ __device__ __forceinline__ void test_apply(const Tables1D& t, FQ&& fq, double* R
           double s = 0.0, z = 0.0;
 #pragma unroll
           for (int qy = 0; qy < NQ1; ++qy) {
-            s = fma(t.B[iy][qy], A0[ix][qy][qz], fma(t.D[iy][qy], A1[ix][qy][qz], s));
-            z = fma(t.B[iy][qy], A2[ix][qy][qz], z);
+            s = fma(t.B[iy][qy], A0[ix][qy], fma(t.D[iy][qy], A1[ix][qy], s));
+            z = fma(t.B[iy][qy], A2[ix][qy], z);
           }
-          S[ix][iy][qz] = s;
-          Z[ix][iy][qz] = z;
+#pragma unroll
+          for (int iz = 0; iz < P1; ++iz) {
+            double& r = R[ix + P1 * (iy + P1 * iz)];
+            r = fma(t.B[iz][qz], s, fma(t.D[iz][qz], z, r));
+          }
         }
-#pragma unroll
-    for (int iz = 0; iz < P1; ++iz)
-#pragma unroll
-      for (int iy = 0; iy < P1; ++iy)
-#pragma unroll
-        for (int ix = 0; ix < P1; ++ix) {
-          double s = 0.0;
-#pragma unroll
-          for (int qz = 0; qz < NQ1; ++qz) s = fma(t.B[iz][qz], S[ix][iy][qz], fma(t.D[iz][qz], Z[ix][iy][qz], s));
-          R[ix + P1 * (iy + P1 * iz)] = s;
-        }
+    }
   }
 }
 
@@ -247,29 +327,80 @@ __device__ __forceinline__ void axis_slot(int il, int jl, int ecd, int nnd, int&
   }
 }
 
-template <int DIM, int P, int NQ1, bool JAC>
+// Element owned by group g of this workgroup in sub-phase `sub`.
+//   element colouring (a.blocked == 0): one launch per colour, one element per
+//     group, elements of the colour enumerated over the grid;
+//   block colouring (a.blocked == 1): the mesh is tiled by blocks of
+//     bw = 2*s elements per axis; one launch per block colour, one block per
+//     workgroup; sub-phase `sub` (an element colour) gives group g the element
+//     block_origin + 2*g_d + sub_d.  Same-colour elements of one block share no
+//     DOF, and same-colour blocks are a block apart, so no two concurrent
+//     element scatters touch the same CRS entry.
+template <int DIM>
+__device__ __forceinline__ bool element_of(const AsmArgs& a, int g, int sub, int* ec) {
+  ec[0] = ec[1] = ec[2] = 0;
+  if (!a.blocked) {
+    const int idx = (int)blockIdx.x * a.gpb + g;
+    int t = idx < a.ncolor ? idx : 0;
+    ec[0] = 2 * (t % a.cn[0]) + (a.color & 1);
+    t /= a.cn[0];
+    ec[1] = 2 * (t % a.cn[1]) + ((a.color >> 1) & 1);
+    t /= a.cn[1];
+    if (DIM == 3) ec[2] = 2 * t + ((a.color >> 2) & 1);
+    return idx < a.ncolor;
+  }
+  int t = blockIdx.x;
+  const int gc[3] = {g % a.s[0], (g / a.s[0]) % a.s[1], g / (a.s[0] * a.s[1])};
+  bool ok = true;
+  for (int d = 0; d < DIM; ++d) {
+    const int bi = 2 * (t % a.cn[d]) + ((a.color >> d) & 1);
+    t /= a.cn[d];
+    ec[d] = bi * a.bw[d] + 2 * gc[d] + ((sub >> d) & 1);
+    ok = ok && ec[d] < a.m.nel[d];
+  }
+  return ok && g < a.gpb;
+}
+
+// Scheduling key of element index e along one axis: (block colour bit,
+// element colour bit).  Launch order is block colour major, sub-phase minor,
+// so along each axis the smaller key runs first.
+__device__ __forceinline__ int axis_key(const AsmArgs& a, int d, int e) {
+  return (a.blocked ? ((e / a.bw[d]) & 1) * 2 : 0) + (e & 1);
+}
+
+template <int DIM, int P, int NQ1, bool JAC, bool BLOCKED>
 __global__ __launch_bounds__(256) void k_thermal(AsmArgs a) {
   using C = Cfg<DIM, P, NQ1>;
   constexpr int NB = C::NB, NQ = C::NQ, NV = C::NV, NC = C::NC, G = C::G, GPB = C::GPB, P1 = C::P1;
+  // rows of Dhat that can be nonzero: the flux rows, plus the source row when
+  // the source depends on the solution
+  constexpr int NR = ThermalPhysics::kSrcDependsOnU ? NC : DIM;
   __shared__ double s_u[GPB][NB];
   __shared__ double s_X[GPB][NV * DIM];
-  __shared__ double s_D[GPB][JAC ? NQ * NC * NC : 1];
+  // Dhat = w T D T^T per point
+  constexpr int NW = JAC ? NQ * NR * NC : 1;
+  __shared__ double s_W[GPB][NW];
   __shared__ double s_F[GPB][NQ * NC];
   __shared__ int s_rp[GPB][NB];
 
   const int g = threadIdx.x / G, lane = threadIdx.x % G;
-  const long idx = (long)blockIdx.x * GPB + g;
-  const bool active = idx < a.ncolor;
-  int ec[3] = {0, 0, 0};
-  {
-    long t = active ? idx : 0;
-    ec[0] = 2 * (int)(t % a.cn[0]) + (a.color & 1);
-    t /= a.cn[0];
-    ec[1] = 2 * (int)(t % a.cn[1]) + ((a.color >> 1) & 1);
-    t /= a.cn[1];
-    if (DIM == 3) ec[2] = 2 * (int)t + ((a.color >> 2) & 1);
-  }
+  // element colouring runs one element per group and no sub-phase loop (a
+  // loop-carried kernel keeps more values live across iterations)
+  constexpr int kMaxSub = BLOCKED ? (1 << DIM) : 1;
+  for (int sub = 0; sub < kMaxSub; ++sub) {
+  if (sub) __syncthreads();  // previous sub-phase's LDS reads and global stores are complete
+  int ec[3];
+  const bool active = element_of<DIM>(a, g, sub, ec);
   const long e = ec[0] + (long)a.m.nel[0] * (ec[1] + (long)a.m.nel[1] * ec[2]);
+  // first-touch flags per axis: is this element the first scheduled one among
+  // the (at most two) elements sharing its low / high vertex plane
+  bool first_lo[3] = {true, true, true}, first_hi[3] = {true, true, true};
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) {
+    const int k = axis_key(a, d, ec[d]);
+    first_lo[d] = ec[d] == 0 || k < axis_key(a, d, ec[d] - 1);
+    first_hi[d] = ec[d] == a.m.nel[d] - 1 || k < axis_key(a, d, ec[d] + 1);
+  }
 
   // 1. gather --------------------------------------------------------------
   if (active) {
@@ -284,6 +415,7 @@ __global__ __launch_bounds__(256) void k_thermal(AsmArgs a) {
   __syncthreads();
 
   // 2. point stage -----------------------------------------------------------
+  constexpr const Tab1& t = T1<P, NQ1>::v;
   if (active) {
     const double k0 = a.coef.k0 ? a.coef.k0[e] : 1.0;
     for (int q = lane; q < NQ; q += G) {
@@ -292,28 +424,35 @@ __global__ __launch_bounds__(256) void k_thermal(AsmArgs a) {
       double uq = 0.0, gu[DIM];
 #pragma unroll
       for (int c = 0; c < DIM; ++c) gu[c] = 0.0;
-      double bq[DIM][P1], dq[DIM][P1];
+      // tensor-product evaluation: the outer basis axis runs as a loop (its
+      // factor read from the table), the inner axes are unrolled
+      double bq[DIM - 1][P1], dq[DIM - 1][P1];
 #pragma unroll
-      for (int d = 0; d < DIM; ++d)
+      for (int d = 0; d < DIM - 1; ++d)
 #pragma unroll
         for (int i = 0; i < P1; ++i) {
-          bq[d][i] = a.t.B[i][qi[d]];
-          dq[d][i] = a.t.D[i][qi[d]];
+          bq[d][i] = t.B[i][qi[d]];
+          dq[d][i] = t.D[i][qi[d]];
         }
+#pragma unroll 1
+      for (int jo = 0; jo < P1; ++jo) {
+        const double bo = t.B[jo][qi[DIM - 1]], dox = t.D[jo][qi[DIM - 1]];
+        constexpr int NI = DIM == 2 ? P1 : P1 * P1;  // inner-plane basis functions
 #pragma unroll
-      for (int j = 0; j < NB; ++j) {
-        const int jl[3] = {j % P1, (j / P1) % P1, DIM == 3 ? j / (P1 * P1) : 0};
-        const double uj = s_u[g][j];
-        double b = 1.0;
-#pragma unroll
-        for (int d = 0; d < DIM; ++d) b *= bq[d][jl[d]];
-        uq = fma(uj, b, uq);
-#pragma unroll
-        for (int c = 0; c < DIM; ++c) {
-          double gg = 1.0;
-#pragma unroll
-          for (int d = 0; d < DIM; ++d) gg *= (d == c) ? dq[d][jl[d]] : bq[d][jl[d]];
-          gu[c] = fma(uj, gg, gu[c]);
+        for (int ji = 0; ji < NI; ++ji) {
+          const int jx = ji % P1, jy = ji / P1;
+          const double uj = s_u[g][ji + NI * jo];
+          if constexpr (DIM == 2) {
+            uq = fma(uj, bq[0][jx] * bo, uq);
+            gu[0] = fma(uj, dq[0][jx] * bo, gu[0]);
+            gu[1] = fma(uj, bq[0][jx] * dox, gu[1]);
+          } else {
+            const double bxy = bq[0][jx] * bq[1][jy];
+            uq = fma(uj, bxy * bo, uq);
+            gu[0] = fma(uj, dq[0][jx] * bq[1][jy] * bo, gu[0]);
+            gu[1] = fma(uj, bq[0][jx] * dq[1][jy] * bo, gu[1]);
+            gu[2] = fma(uj, bxy * dox, gu[2]);
+          }
         }
       }
       // geometry: x(xi) = sum_v N_v X_v,  J_ab = dx_a/dxi_b
@@ -322,12 +461,20 @@ __global__ __launch_bounds__(256) void k_thermal(AsmArgs a) {
       for (int r = 0; r < DIM; ++r)
 #pragma unroll
         for (int s = 0; s < DIM; ++s) J[r][s] = 0.0;
+      double nq[DIM][2], mq[DIM][2];
+#pragma unroll
+      for (int d = 0; d < DIM; ++d)
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+          nq[d][i] = t.N[i][qi[d]];
+          mq[d][i] = t.M[i][qi[d]];
+        }
 #pragma unroll
       for (int v = 0; v < NV; ++v) {
         const int vl[3] = {v & 1, (v >> 1) & 1, (v >> 2) & 1};
         double nv = 1.0;
 #pragma unroll
-        for (int d = 0; d < DIM; ++d) nv *= a.t.N[vl[d]][qi[d]];
+        for (int d = 0; d < DIM; ++d) nv *= nq[d][vl[d]];
 #pragma unroll
         for (int r = 0; r < DIM; ++r) {
           const double X = s_X[g][v * DIM + r];
@@ -336,7 +483,7 @@ __global__ __launch_bounds__(256) void k_thermal(AsmArgs a) {
           for (int s = 0; s < DIM; ++s) {
             double dn = 1.0;
 #pragma unroll
-            for (int d = 0; d < DIM; ++d) dn *= (d == s) ? a.t.M[vl[d]][qi[d]] : a.t.N[vl[d]][qi[d]];
+            for (int d = 0; d < DIM; ++d) dn *= (d == s) ? mq[d][vl[d]] : nq[d][vl[d]];
             J[r][s] = fma(X, dn, J[r][s]);
           }
         }
@@ -367,7 +514,7 @@ __global__ __launch_bounds__(256) void k_thermal(AsmArgs a) {
       }
       double w = fabs(det);
 #pragma unroll
-      for (int d = 0; d < DIM; ++d) w *= a.t.W[qi[d]];
+      for (int d = 0; d < DIM; ++d) w *= t.W[qi[d]];
       // physical gradient
       double gp[DIM];
 #pragma unroll
@@ -396,8 +543,8 @@ __global__ __launch_bounds__(256) void k_thermal(AsmArgs a) {
       }
       F[DIM] = w * src.v;
       if constexpr (JAC) {
-        // TD[r][c'] = sum_c T[r][c] D[c][c']
-        double TD[NC][NC];
+        // TD[r][c'] = sum_c T[r][c] D[c][c']  (only the NR rows the physics can make nonzero)
+        double TD[NR][NC];
 #pragma unroll
         for (int cc = 0; cc < NC; ++cc) {
 #pragma unroll
@@ -407,11 +554,11 @@ __global__ __launch_bounds__(256) void k_thermal(AsmArgs a) {
             for (int c = 0; c < DIM; ++c) s = fma(Ji[r][c], flux[c].d[cc], s);
             TD[r][cc] = s;
           }
-          TD[DIM][cc] = src.d[cc];
+          if constexpr (NR > DIM) TD[NR - 1][cc] = src.d[cc];
         }
-        double* Dh = &s_D[g][q * NC * NC];
+        double* Dh = &s_W[g][q * NR * NC];
 #pragma unroll
-        for (int r = 0; r < NC; ++r) {
+        for (int r = 0; r < NR; ++r) {
 #pragma unroll
           for (int s2 = 0; s2 < DIM; ++s2) {
             double s = 0.0;
@@ -430,39 +577,43 @@ __global__ __launch_bounds__(256) void k_thermal(AsmArgs a) {
   double R[NB];
 #pragma unroll
   for (int i = 0; i < NB; ++i) R[i] = 0.0;
-  const double* Dg = s_D[g];
+  const double* Dg = s_W[g];
   const double* Fg = s_F[g];
   if constexpr (JAC) {
     if (active && lane < NB) {
       const int jl[3] = {lane % P1, (lane / P1) % P1, DIM == 3 ? lane / (P1 * P1) : 0};
-      double bj[DIM][NQ1], dj[DIM][NQ1];
+      // the lane's 1-D factors along the two inner axes; the outer-axis factor
+      // is read per plane
+      double bj[2][NQ1], dj[2][NQ1];
 #pragma unroll
-      for (int d = 0; d < DIM; ++d)
+      for (int d = 0; d < 2; ++d)
 #pragma unroll
         for (int q = 0; q < NQ1; ++q) {
-          bj[d][q] = a.t.B[jl[d]][q];
-          dj[d][q] = a.t.D[jl[d]][q];
+          bj[d][q] = d < DIM - 1 ? t.B[jl[d]][q] : 0.0;
+          dj[d][q] = d < DIM - 1 ? t.D[jl[d]][q] : 0.0;
         }
       test_apply<DIM, P, NQ1>(
-          a.t,
-          [&](int q, double* f) {
-            const int qi[3] = {q % NQ1, (q / NQ1) % NQ1, DIM == 3 ? q / (NQ1 * NQ1) : 0};
+          t,
+          [&](int qx, int qy, int qz, double* f) {
+            // outer axis (y in 2-D, z in 3-D) index and factor
+            const int qo = DIM == 2 ? qy : qz;
+            const double bo = t.B[jl[DIM - 1]][qo], dox = t.D[jl[DIM - 1]][qo];
             double gh[NC];
-            double b = 1.0;
-#pragma unroll
-            for (int d = 0; d < DIM; ++d) b *= bj[d][qi[d]];
-            gh[DIM] = b;
-#pragma unroll
-            for (int c = 0; c < DIM; ++c) {
-              double gg = 1.0;
-#pragma unroll
-              for (int d = 0; d < DIM; ++d) gg *= (d == c) ? dj[d][qi[d]] : bj[d][qi[d]];
-              gh[c] = gg;
+            if constexpr (DIM == 2) {
+              gh[0] = dj[0][qx] * bo;
+              gh[1] = bj[0][qx] * dox;
+              gh[2] = bj[0][qx] * bo;
+            } else {
+              gh[0] = dj[0][qx] * bj[1][qy] * bo;
+              gh[1] = bj[0][qx] * dj[1][qy] * bo;
+              gh[2] = bj[0][qx] * bj[1][qy] * dox;
+              gh[3] = bj[0][qx] * bj[1][qy] * bo;
             }
-            const double* Dh = Dg + q * NC * NC;
+            const int q = DIM == 2 ? qx + NQ1 * qy : qx + NQ1 * (qy + NQ1 * qz);
+            const double* Dh = Dg + q * NR * NC;
 #pragma unroll
             for (int r = 0; r < NC; ++r) {
-              if (r == DIM && !ThermalPhysics::kSrcDependsOnU) {
+              if (r >= NR) {
                 f[r] = 0.0;
                 continue;
               }
@@ -475,8 +626,9 @@ __global__ __launch_bounds__(256) void k_thermal(AsmArgs a) {
           R);
     } else if (active && lane == NB && a.res) {
       test_apply<DIM, P, NQ1>(
-          a.t,
-          [&](int q, double* f) {
+          t,
+          [&](int qx, int qy, int qz, double* f) {
+            const int q = DIM == 2 ? qx + NQ1 * qy : qx + NQ1 * (qy + NQ1 * qz);
 #pragma unroll
             for (int r = 0; r < NC; ++r) f[r] = Fg[q * NC + r];
           },
@@ -490,13 +642,13 @@ __global__ __launch_bounds__(256) void k_thermal(AsmArgs a) {
         const int qi[3] = {q % NQ1, (q / NQ1) % NQ1, DIM == 3 ? q / (NQ1 * NQ1) : 0};
         double b = 1.0;
 #pragma unroll
-        for (int d = 0; d < DIM; ++d) b *= a.t.B[il[d]][qi[d]];
+        for (int d = 0; d < DIM; ++d) b *= t.B[il[d]][qi[d]];
         s = fma(Fg[q * NC + DIM], b, s);
 #pragma unroll
         for (int c = 0; c < DIM; ++c) {
           double gg = 1.0;
 #pragma unroll
-          for (int d = 0; d < DIM; ++d) gg *= (d == c) ? a.t.D[il[d]][qi[d]] : a.t.B[il[d]][qi[d]];
+          for (int d = 0; d < DIM; ++d) gg *= (d == c) ? t.D[il[d]][qi[d]] : t.B[il[d]][qi[d]];
           s = fma(Fg[q * NC + c], gg, s);
         }
       }
@@ -506,8 +658,9 @@ __global__ __launch_bounds__(256) void k_thermal(AsmArgs a) {
   __syncthreads();  // all reads of s_u done before it is reused
 
   // 4. scatter -------------------------------------------------------------------
-  // lane i fetches row i's CRS offset (or -1 for a strong-Dirichlet row) once;
-  // the value lane hands the residual entries to the row lanes through LDS.
+  // Lane i fetches row i's CRS offset once (negated minus one for a
+  // strong-Dirichlet row); the value lane hands the residual to the row lanes
+  // through s_u.
   if constexpr (JAC) {
     if (active && lane == NB && a.res) {
 #pragma unroll
@@ -518,35 +671,78 @@ __global__ __launch_bounds__(256) void k_thermal(AsmArgs a) {
   }
   if (active && lane < NB) {
     const long row = node_of<DIM, P>(a.m, ec, lane);
-    s_rp[g][lane] = a.fixed[row] ? -1 : a.rowptr[row];
+    s_rp[g][lane] = a.fixed[row] ? -1 - a.rowptr[row] : a.rowptr[row];
   }
   __syncthreads();
+  // An entry (row node r, column node c) is shared by the elements holding
+  // both nodes: along each axis one element, or two when r and c sit on the
+  // same vertex plane.  With a.first_touch the first scheduled of them writes
+  // the entry (0 / 1 on strong-Dirichlet rows) and later ones add into it, so
+  // no zero-fill pass and no separate dofConstraints pass are needed.
+  // Otherwise every contribution is added and Dirichlet rows are skipped
+  // (the caller zeroes; assemblyManager_scatter.hpp:208,253 semantics).
+  auto shared_first = [&](const int* il, const int* jl) {
+    bool f = true;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d)
+      if (il[d] == jl[d] && (il[d] == 0 || il[d] == P)) f = f && (il[d] == 0 ? first_lo[d] : first_hi[d]);
+    return f;
+  };
   if (active && lane < NB && a.res) {
     const long row = node_of<DIM, P>(a.m, ec, lane);
-    if (s_rp[g][lane] >= 0) a.res[row] += -s_u[g][lane];
+    const int il[3] = {lane % P1, (lane / P1) % P1, DIM == 3 ? lane / (P1 * P1) : 0};
+    const bool fx = s_rp[g][lane] < 0;
+    const double v = -s_u[g][lane];
+    if (a.first_touch && shared_first(il, il)) a.res[row] = fx ? 0.0 : v;
+    else if (!fx) a.res[row] += v;
   }
   if constexpr (JAC) {
     if (active && lane < NB && a.jac) {
+      // lane j holds column j; rows are unrolled so the per-axis slot logic
+      // folds at compile time, and for a fixed row the group's lanes write the
+      // row's runs of consecutive slots.  Per chunk of rows: slots and flags
+      // first, the loads that are needed (later touches of non-Dirichlet
+      // rows) issued together, then the stores.
       const int jl[3] = {lane % P1, (lane / P1) % P1, DIM == 3 ? lane / (P1 * P1) : 0};
-      // all loads of the column first (independent, in flight together), then the stores
-      int sl[NB];
-      double old[NB];
+      constexpr int CH = NB < 9 ? NB : 9;
 #pragma unroll
-      for (int i = 0; i < NB; ++i) {
-        const int il[3] = {i % P1, (i / P1) % P1, DIM == 3 ? i / (P1 * P1) : 0};
-        int px, lx, py, ly, pz = 0, lz = 1;
-        axis_slot<P>(il[0], jl[0], ec[0], a.m.nn[0], px, lx);
-        axis_slot<P>(il[1], jl[1], ec[1], a.m.nn[1], py, ly);
-        if (DIM == 3) axis_slot<P>(il[2], jl[2], ec[2], a.m.nn[2], pz, lz);
-        const int rp = s_rp[g][i];
-        sl[i] = rp >= 0 ? rp + (pz * ly + py) * lx + px : -1;
-        old[i] = a.jac[sl[i] >= 0 ? sl[i] : 0];
+      for (int i0 = 0; i0 < NB; i0 += CH) {
+        int sl[CH];
+        uint32_t first_mask = 0, fixed_mask = 0;
+        double old[CH];
+#pragma unroll
+        for (int k = 0; k < CH; ++k) {
+          const int i = i0 + k;
+          if (i >= NB) break;
+          const int il[3] = {i % P1, (i / P1) % P1, DIM == 3 ? i / (P1 * P1) : 0};
+          int px, lx, py, ly, pz = 0, lz = 1;
+          axis_slot<P>(il[0], jl[0], ec[0], a.m.nn[0], px, lx);
+          axis_slot<P>(il[1], jl[1], ec[1], a.m.nn[1], py, ly);
+          if (DIM == 3) axis_slot<P>(il[2], jl[2], ec[2], a.m.nn[2], pz, lz);
+          int rp = s_rp[g][i];
+          const bool fx = rp < 0;
+          if (fx) rp = -1 - rp;
+          sl[k] = rp + (pz * ly + py) * lx + px;
+          const bool first = a.first_touch && shared_first(il, jl);
+          if (first) first_mask |= 1u << k;
+          if (fx) fixed_mask |= 1u << k;
+          old[k] = (!first && !fx) ? a.jac[sl[k]] : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < CH; ++k) {
+          const int i = i0 + k;
+          if (i >= NB) break;
+          const bool first = (first_mask >> k) & 1u, fx = (fixed_mask >> k) & 1u;
+          if (fx) {
+            if (first) a.jac[sl[k]] = (i == lane) ? 1.0 : 0.0;
+          } else {
+            a.jac[sl[k]] = old[k] + R[i];
+          }
+        }
       }
-#pragma unroll
-      for (int i = 0; i < NB; ++i)
-        if (sl[i] >= 0) a.jac[sl[i]] = old[i] + R[i];
     }
   }
+  }  // sub-phase loop
 }
 
 // strong Dirichlet rows: J(d,d) = 1 (setJacobianConstraints, constraints.hpp:125-141)
@@ -567,28 +763,56 @@ __global__ void k_dbc_diag(BoxMesh m, const int32_t* __restrict__ list, long n, 
 // ---------------------------------------------------------------------------
 // host launch
 // ---------------------------------------------------------------------------
+// groups per axis for block colouring: split GPB over the axes, doubling the
+// currently smallest one (x first)
+static void block_shape(int dim, int gpb, int* s) {
+  s[0] = s[1] = s[2] = 1;
+  int prod = 1;
+  while (prod < gpb) {
+    int d = 0;
+    for (int k = 1; k < dim; ++k)
+      if (s[k] < s[d]) d = k;
+    s[d] *= 2;
+    prod *= 2;
+  }
+}
+
 template <int DIM, int P, int NQ1>
 static void launch_colors(Ctx& c, AsmArgs& a, bool jac) {
   using C = Cfg<DIM, P, NQ1>;
   const int ncol = 1 << DIM;
+  a.gpb = C::GPB;
+  a.blocked = c.scatter_mode == kBlockColouring;
+  a.nsub = a.blocked ? ncol : 1;
+  block_shape(DIM, C::GPB, a.s);
+  for (int d = 0; d < 3; ++d) a.bw[d] = 2 * a.s[d];
   for (int col = 0; col < ncol; ++col) {
     a.color = col;
     long cnt = 1;
     for (int d = 0; d < 3; ++d) {
-      int cd = (col >> d) & 1;
-      int n = d < DIM ? (c.mesh.nel[d] - cd + 1) / 2 : 1;
+      const int cd = (col >> d) & 1;
+      // units along the axis: elements (element colouring) or blocks (block colouring)
+      const int units = d < DIM ? (a.blocked ? (c.mesh.nel[d] + a.bw[d] - 1) / a.bw[d] : c.mesh.nel[d]) : 1;
+      int n = d < DIM ? (units - cd + 1) / 2 : 1;
       if (d >= DIM && cd) n = 0;
       a.cn[d] = n;
       cnt *= n;
     }
     a.ncolor = cnt;
     if (cnt == 0) continue;
-    long blocks = (cnt + C::GPB - 1) / C::GPB;
+    const long blocks = a.blocked ? cnt : (cnt + C::GPB - 1) / C::GPB;
     ScopedPhase ph(c.prof, jac ? "k_thermal<jacres>" : "k_thermal<res>", c.stream);
-    if (jac)
-      hipLaunchKernelGGL((k_thermal<DIM, P, NQ1, true>), dim3((unsigned)blocks), dim3(C::BLOCK), 0, c.stream, a);
-    else
-      hipLaunchKernelGGL((k_thermal<DIM, P, NQ1, false>), dim3((unsigned)blocks), dim3(C::BLOCK), 0, c.stream, a);
+    if (a.blocked) {
+      if (jac)
+        hipLaunchKernelGGL((k_thermal<DIM, P, NQ1, true, true>), dim3((unsigned)blocks), dim3(C::BLOCK), 0, c.stream, a);
+      else
+        hipLaunchKernelGGL((k_thermal<DIM, P, NQ1, false, true>), dim3((unsigned)blocks), dim3(C::BLOCK), 0, c.stream, a);
+    } else {
+      if (jac)
+        hipLaunchKernelGGL((k_thermal<DIM, P, NQ1, true, false>), dim3((unsigned)blocks), dim3(C::BLOCK), 0, c.stream, a);
+      else
+        hipLaunchKernelGGL((k_thermal<DIM, P, NQ1, false, false>), dim3((unsigned)blocks), dim3(C::BLOCK), 0, c.stream, a);
+    }
     MHX_HIP(hipGetLastError());
   }
 }
@@ -621,7 +845,6 @@ void assemble_thermal(Ctx& c, const double* u, double* res, double* jac, bool wa
   MHX_REQUIRE(f, MHX_ERR_UNSUPPORTED, "no kernel built for this dim/order/quadrature");
   AsmArgs a{};
   a.m = c.mesh;
-  a.t = c.tab;
   a.coef.k0 = c.k0.p;
   a.coef.src = c.src.p;
   a.coef.diff_u2 = c.desc.diff_u2;
@@ -636,18 +859,15 @@ void assemble_thermal(Ctx& c, const double* u, double* res, double* jac, bool wa
   a.fixed = c.fixed.p;
   a.res = want_res ? res : nullptr;
   a.jac = want_jac ? jac : nullptr;
-  {
-    ScopedPhase ph(c.prof, "AssemblyManager::zero", c.stream);
-    if (!accumulate) {
-      if (want_res) MHX_HIP(hipMemsetAsync(res, 0, c.mesh.ndof * sizeof(double), c.stream));
-      if (want_jac) MHX_HIP(hipMemsetAsync(jac, 0, c.nnz * sizeof(double), c.stream));
-    }
-  }
+  // Overwrite mode: the first scheduled element touching an entry writes it,
+  // which also writes the Dirichlet rows (0, diagonal 1).  Accumulate mode:
+  // contributions are added and the Dirichlet diagonal is set afterwards.
+  a.first_touch = accumulate ? 0 : 1;
   {
     ScopedPhase ph(c.prof, want_jac ? "AssemblyManager::assembleJacRes" : "AssemblyManager::assembleRes", c.stream);
     f(c, a, want_jac);
   }
-  if (want_jac && !c.h_dbc_list.empty()) {
+  if (accumulate && want_jac && !c.h_dbc_list.empty()) {
     ScopedPhase ph(c.prof, "AssemblyManager::dofConstraints", c.stream);
     long n = (long)c.h_dbc_list.size();
     hipLaunchKernelGGL(k_dbc_diag, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream, c.mesh, c.dbc_list.p, n,

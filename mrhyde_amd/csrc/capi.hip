@@ -211,6 +211,16 @@ __global__ __launch_bounds__(256) void k_l2err(ErrArgs a) {
   }
 }
 
+// plain copy, two doubles per thread per iteration (calibration kernel)
+__global__ __launch_bounds__(256) void k_copy_probe(int64_t n, const double* __restrict__ x, double* __restrict__ y) {
+  const int64_t n2 = n / 2;
+  const double2* x2 = reinterpret_cast<const double2*>(x);
+  double2* y2 = reinterpret_cast<double2*>(y);
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += (int64_t)gridDim.x * blockDim.x)
+    y2[i] = x2[i];
+  if (blockIdx.x == 0 && threadIdx.x == 0 && (n & 1)) y[n - 1] = x[n - 1];
+}
+
 // ---- Newton (SolverManager::nonlinearSolver, solverManager_solvers.hpp:290-643) --
 static int newton(Ctx& c, double* u, const mhx_newton_opts& o, mhx_solve_stats* st) {
   const long n = c.mesh.ndof;
@@ -577,6 +587,31 @@ int mhx_synchronize(mhx_ctx* ctx) {
   return MHX_OK;
 }
 
+int mhx_set_option(mhx_ctx* ctx, const char* name, int64_t value) {
+  if (!ctx || !name) return MHX_ERR_ARG;
+  const std::string k(name);
+  if (k == "assembly.colouring") {
+    if (value != kElementColouring && value != kBlockColouring) {
+      ctx->err = "assembly.colouring must be 0 or 1";
+      return MHX_ERR_ARG;
+    }
+    ctx->scatter_mode = (int)value;
+    return MHX_OK;
+  }
+  ctx->err = "unknown option " + k;
+  return MHX_ERR_ARG;
+}
+
+int mhx_get_option(const mhx_ctx* ctx, const char* name, int64_t* value) {
+  if (!ctx || !name || !value) return MHX_ERR_ARG;
+  const std::string k(name);
+  if (k == "assembly.colouring") {
+    *value = ctx->scatter_mode;
+    return MHX_OK;
+  }
+  return MHX_ERR_ARG;
+}
+
 static DevBuf<double>& state_buf(mhx_ctx* ctx, int32_t which, size_t& count) {
   switch (which) {
     case MHX_BUF_U: count = ctx->mesh.ndof; return ctx->u;
@@ -643,6 +678,37 @@ int mhx_set_stream(mhx_ctx* ctx, void* hip_stream) {
   }
   ctx->stream = (hipStream_t)hip_stream;
   return MHX_OK;
+}
+
+int mhx_copy_probe(int64_t n_doubles, int32_t reps, double* ms_per_copy) {
+  try {
+    if (n_doubles <= 0 || reps <= 0 || !ms_per_copy) return MHX_ERR_ARG;
+    DevBuf<double> x, y;
+    x.alloc(n_doubles);
+    y.alloc(n_doubles);
+    hipStream_t s;
+    MHX_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    MHX_HIP(hipMemsetAsync(x.p, 0, n_doubles * sizeof(double), s));
+    const unsigned grid = (unsigned)std::min<int64_t>((n_doubles / 2 + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_copy_probe, dim3(grid), dim3(256), 0, s, n_doubles, x.p, y.p);
+    hipEvent_t a, b;
+    MHX_HIP(hipEventCreate(&a));
+    MHX_HIP(hipEventCreate(&b));
+    MHX_HIP(hipEventRecord(a, s));
+    for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(k_copy_probe, dim3(grid), dim3(256), 0, s, n_doubles, x.p, y.p);
+    MHX_HIP(hipEventRecord(b, s));
+    MHX_HIP(hipEventSynchronize(b));
+    float ms = 0.f;
+    MHX_HIP(hipEventElapsedTime(&ms, a, b));
+    *ms_per_copy = ms / reps;
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    (void)hipStreamDestroy(s);
+    return MHX_OK;
+  } catch (const Error& e) {
+    g_last_error = e.msg;
+    return e.code;
+  }
 }
 
 static inline uint64_t splitmix(uint64_t seed, uint64_t idx) {

@@ -70,7 +70,6 @@ struct ThermalCoef {
 
 struct AsmArgs {
   BoxMesh m;
-  Tables1D t;
   ThermalCoef coef;
   const double* vx;       // [nvert*3] vertex coordinates
   const double* u;        // [ndof]
@@ -78,10 +77,18 @@ struct AsmArgs {
   const uint8_t* fixed;   // [ndof]
   double* res;            // [ndof] or nullptr
   double* jac;            // [nnz]  or nullptr
-  int color;              // colour bits (cx | cy<<1 | cz<<2)
-  long ncolor;            // elements of this colour
-  int cn[3];              // elements per axis inside this colour
+  int color;              // colour bits (cx | cy<<1 | cz<<2): element or block colour
+  long ncolor;            // elements (element colouring) / blocks (block colouring) of this colour
+  int cn[3];              // per-axis count inside this colour
+  int blocked;            // 0: element colouring, 1: block colouring
+  int nsub;               // sub-phases per workgroup (1, or 2^dim element colours)
+  int gpb;                // element groups per workgroup
+  int s[3];               // block colouring: groups per axis (prod = gpb)
+  int bw[3];              // block colouring: block width in elements (2*s)
+  int first_touch;        // overwrite on first touch instead of accumulate
 };
+
+enum ScatterMode { kElementColouring = 0, kBlockColouring = 1 };
 
 // ---------------------------------------------------------------------------
 // device buffers
@@ -189,6 +196,7 @@ struct Ctx {
   bool own_stream = false;
   Profiler prof;
   int spmv_lanes = 16;
+  int scatter_mode = kElementColouring;
   std::string err;
 };
 

@@ -113,6 +113,15 @@ class Mhx:
             raise MhxError(rc, f"{what}: {lib.mhx_last_error(self._h).decode()}")
         return rc
 
+    # ---- options ------------------------------------------------------------------
+    def set_option(self, name: str, value: int):
+        self._check(lib.mhx_set_option(self._h, name.encode(), int(value)), f"set_option({name})")
+
+    def get_option(self, name: str) -> int:
+        v = C.c_int64()
+        self._check(lib.mhx_get_option(self._h, name.encode(), C.byref(v)), f"get_option({name})")
+        return v.value
+
     # ---- device-resident state ------------------------------------------------
     def state_ptr(self, which: int):
         """Device address (int) and length of a context-owned buffer (MHX_BUF_*)."""

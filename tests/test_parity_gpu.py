@@ -73,29 +73,38 @@ def test_pattern_and_dirichlet_bit_exact(pairs, name):
     np.testing.assert_allclose(g.dof_coords(), o.dof_coords(), rtol=0, atol=1e-15)
 
 
+@pytest.mark.parametrize("colouring", [1, 0])
 @pytest.mark.parametrize("name", list(CONFIGS))
-def test_assembly_parity(pairs, name):
+def test_assembly_parity(pairs, name, colouring):
     g, o = pairs[name]
-    print(f"[parity] {name} n_dof={g.n_dof} nnz={g.nnz}", flush=True)
+    print(f"[parity] {name} colouring={colouring} n_dof={g.n_dof} nnz={g.nnz}", flush=True)
+    g.set_option("assembly.colouring", colouring)
     u = random_state(o, 17)
-    rg, jg = g.assembleJacRes(u)
+    # stale values in the output buffers must not leak into an overwrite-mode assembly
+    rg = np.full(g.n_dof, 7.0)
+    jg = np.full(g.nnz, -3.0)
+    g.assemble(u, rg, jg)
     ro, jo = o.assembleJacRes(u)
     rp, _ = o.crs_pattern()
     assert_entries_close(jg, jo, rp, what=f"{name} Jacobian")
     assert_entries_close(rg, ro, None, what=f"{name} residual")
     # residual-only pass (assembleRes) gives the same residual
     assert_entries_close(g.assembleRes(u), ro, None, what=f"{name} residual-only")
+    g.set_option("assembly.colouring", 1)
 
 
-def test_accumulate_and_device_pointers(pairs):
+@pytest.mark.parametrize("colouring", [1, 0])
+def test_accumulate_and_device_pointers(pairs, colouring):
     from mrhyde_amd.abi import MHX_BUF_JAC, MHX_BUF_RES, MHX_BUF_U
 
     g, o = pairs["3d_q2_c2shape"]
+    g.set_option("assembly.colouring", colouring)
     u = random_state(o, 3)
     ro, jo = o.assembleJacRes(u)
     g.state_upload(MHX_BUF_U, u)
     g.assemble_resident()
     g.assemble_resident(accumulate=True)
+    g.set_option("assembly.colouring", 1)
     rp, ci = o.crs_pattern()
     mask, _ = o.dirichlet()
     jo2 = 2 * jo
