@@ -62,6 +62,51 @@ def cpu_baseline(n_full: int, sample_n: int = 24):
                       f"{dt:.2f} s), scaled per element to {n_full}^3"}
 
 
+def subgrid_bench(steps: int, per_call_sample: int = 16):
+    """4096 independent Q1 thermal subproblems (n in {4,6,8,12,16}, lognormal
+    k with sigma 1, Dirichlet data U[-1,1]) assembled and solved to 1e-10 in
+    one persistent launch replayed from a hipGraph; FP64 and FP32-inner /
+    FP64-refined; plus the plain per-call engine path timed on a sample."""
+    from mrhyde_amd import Mhx, SubgridBatch
+    from mrhyde_amd.abi import Problem, subgrid_workload
+
+    n_axis, k, g = subgrid_workload()
+    out = {"workload": "4096 unit-cube Q1 thermal subproblems, n^3 elements, n in {4,6,8,12,16}, "
+                       "lognormal k (sigma 1), random Dirichlet data, Jacobi-PCG to 1e-10",
+           "n_instances": int(len(n_axis)), "total_nodes": int(((n_axis.astype(np.int64) + 1) ** 3).sum())}
+    for prec in ("fp64", "fp32-refined"):
+        b = SubgridBatch(n_axis, k, g, tol=1e-10, precision=prec, use_graph=True)
+        b.solve()
+        ms = [b.solve() for _ in range(steps)]
+        _, it, rel, refs = b.results()
+        t = float(np.mean(ms))
+        out[prec] = {"ms_per_batch": round(t, 3), "solves_per_s": round(len(n_axis) / (t * 1e-3), 1),
+                     "max_relres": float(rel.max()), "mean_iters": float(it.mean()), "wg_size": b.wg_size,
+                     "mean_refinements": float(refs.mean())}
+        b.close()
+    # plain per-call path: one engine context + J/R assembly + Jacobi-CG per instance
+    ko = np.concatenate([[0], np.cumsum(n_axis.astype(np.int64) ** 3)])
+    xo = np.concatenate([[0], np.cumsum((n_axis.astype(np.int64) + 1) ** 3)])
+    idx = list(range(0, len(n_axis), max(1, len(n_axis) // per_call_sample)))[:per_call_sample]
+    t_all = 0.0
+    for i in idx:
+        n = int(n_axis[i])
+        t0 = time.perf_counter()
+        ctx = Mhx(Problem(dim=3, order=1, quadrature=2, n=(n, n, n), k0=k[ko[i]:ko[i + 1]],
+                          dbc_values=g[xo[i]:xo[i + 1]]))
+        u = ctx.setDirichlet(np.zeros(ctx.n_dof))
+        res, jac = ctx.assembleJacRes(u)
+        du, _ = ctx.linearSolver(jac, res, solver="cg", tol=1e-10, max_iters=20000)
+        t_all += time.perf_counter() - t0
+        ctx.close()
+    per = t_all / len(idx)
+    out["per_call"] = {"ms_per_instance": round(per * 1e3, 3), "solves_per_s": round(1.0 / per, 1),
+                       "sample": f"{len(idx)} instances spread over the batch (context set-up included)"}
+    out["speedup_fp64_vs_per_call"] = round(out["fp64"]["solves_per_s"] / out["per_call"]["solves_per_s"], 1)
+    out["speedup_fp32r_vs_per_call"] = round(out["fp32-refined"]["solves_per_s"] / out["per_call"]["solves_per_s"], 1)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -71,6 +116,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--colouring", type=int, default=0, help="0 element colouring, 1 block colouring")
     ap.add_argument("--probe-gib", type=float, default=1.0, help="size of the plain-copy bandwidth probe")
+    ap.add_argument("--skip-subgrid", action="store_true", help="skip the batched-subgrid measurement")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
@@ -133,6 +179,8 @@ def main():
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(n)
 
+    subgrid = None if args.skip_subgrid else subgrid_bench(max(3, min(args.steps, 10)))
+
     out = {
         "metric": "assembly MDOF/s (% HBM peak); subgrid solves/s; p50/p99 solve latency @1e-10",
         "value": round(mdofs, 2),
@@ -156,6 +204,7 @@ def main():
                      "colouring": "block" if args.colouring else "element"},
         "cpu_baseline": cpu,
         "phases_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in prof.items()},
+        "subgrid": subgrid,
     }
     print(json.dumps(out), flush=True)
 

@@ -257,6 +257,43 @@ int  mhx_set_stream(mhx_ctx* ctx, void* hip_stream);
 #define MHX_RNG_NORMAL    2
 int  mhx_rng_fill(uint64_t seed, int64_t n, int32_t kind, double a, double b, double* out);
 
+/* ---- batched multiscale subgrid solves ----------------------------------
+ * Many independent thermal subproblems, each the unit cube cut into n^3 Q1
+ * hexahedra (2 <= n <= 16, 2x2x2 Gauss), element conductivities k, strong
+ * Dirichlet data on every boundary node, no source; each is assembled and
+ * solved with Jacobi-PCG to ||b - A x|| / ||b|| <= tol inside ONE persistent
+ * launch (optionally replayed from a captured hipGraph).  FP32_REFINED runs
+ * the inner PCG in FP32 inside FP64 iterative refinement (FP64 residuals).
+ * Replaces the sequential per-macro-group subgrid loop:
+ * SubGridDtN_Solver::nonlinearSolver (src/subgrid/subgridDtN_solver.cpp:903-1043)
+ * called from MultiscaleManager::evaluateMacroMicroMacroMap
+ * (src/managers/multiscale/multiscaleManager.cpp:866-1188). */
+#define MHX_BATCH_FP64          0
+#define MHX_BATCH_FP32_REFINED  1
+typedef struct mhx_batch mhx_batch;
+typedef struct mhx_batch_desc {
+  int32_t n_inst;
+  const int32_t* n_axis;   /* [n_inst] elements per axis */
+  const double* k;         /* concatenated n^3 element conductivities (host) */
+  const double* g;         /* concatenated (n+1)^3 nodal arrays; boundary entries = Dirichlet data (host) */
+  double  tol;             /* default 1e-10 */
+  int32_t max_iters;       /* per instance, default 10000 */
+  int32_t precision;       /* MHX_BATCH_* */
+  double  inner_tol;       /* FP32 inner relative tolerance, default 1e-5 */
+  int32_t max_refine;      /* default 20 */
+  int32_t wg_size;         /* 512 or 1024; 0 = autotune on this batch */
+  int32_t use_graph;       /* 1 = capture the launch once and replay it */
+  int32_t reserved[4];
+} mhx_batch_desc;
+int  mhx_batch_create(const mhx_batch_desc* desc, mhx_batch** out);
+int  mhx_batch_solve(mhx_batch* b, int32_t flags);
+/* Host copies: x (concatenated nodal solutions), iterations, FP64 relative
+ * residual and refinement sweeps per instance (any pointer may be NULL). */
+int  mhx_batch_results(mhx_batch* b, double* x, int32_t* iters, double* relres, int32_t* refinements);
+int  mhx_batch_info(const mhx_batch* b, int64_t* total_nodes, int32_t* wg_size, double* last_ms);
+int  mhx_batch_set_wg(mhx_batch* b, int32_t wg_size);
+void mhx_batch_destroy(mhx_batch* b);
+
 /* Plain device copy y = x of n_doubles FP64 values, repeated reps times on a
  * private stream; returns the mean milliseconds per copy.  A calibration
  * kernel of known traffic (8 B read + 8 B written per value) for HBM

@@ -80,6 +80,31 @@ class SolveStats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+MHX_BATCH_FP64, MHX_BATCH_FP32_REFINED = 0, 1
+
+
+class BatchDesc(C.Structure):
+    _fields_ = [
+        ("n_inst", C.c_int32), ("n_axis", C.POINTER(C.c_int32)), ("k", _dp), ("g", _dp),
+        ("tol", C.c_double), ("max_iters", C.c_int32), ("precision", C.c_int32), ("inner_tol", C.c_double),
+        ("max_refine", C.c_int32), ("wg_size", C.c_int32), ("use_graph", C.c_int32), ("reserved", C.c_int32 * 4),
+    ]
+
+
+def subgrid_workload(n_inst: int = 4096, seed: int = 4096, sizes=(4, 6, 8, 12, 16), sigma: float = 1.0):
+    """The batched-subgrid input set: per instance n ~ U(sizes), element
+    conductivity exp(sigma z), z ~ N(0,1), and Dirichlet data U[-1,1] on the
+    boundary nodes (interior entries are ignored), all from the counter-based
+    generator with the given seed."""
+    pick = rng_fill(seed, n_inst, MHX_RNG_UNIFORM, 0.0, float(len(sizes)))
+    n_axis = np.array([sizes[min(int(p), len(sizes) - 1)] for p in pick], dtype=np.int32)
+    ne = (n_axis.astype(np.int64)) ** 3
+    nn = (n_axis.astype(np.int64) + 1) ** 3
+    k = rng_fill(seed + 1, int(ne.sum()), MHX_RNG_LOGNORMAL, 0.0, sigma)
+    g = rng_fill(seed + 2, int(nn.sum()), MHX_RNG_UNIFORM, -1.0, 1.0)
+    return n_axis, k, g
+
+
 def rng_fill(seed: int, n: int, kind: int = MHX_RNG_UNIFORM, a: float = 0.0, b: float = 1.0) -> np.ndarray:
     """The seeded counter-based generator of mhx_rng_fill, vectorised in numpy:
     u_i = top 53 bits of splitmix64(seed ^ (i*phi + c)) / 2^53; normals by

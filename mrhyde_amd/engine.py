@@ -284,3 +284,70 @@ class Mhx:
 
     def synchronize(self):
         self._check(lib.mhx_synchronize(self._h), "synchronize")
+
+
+class SubgridBatch:
+    """Batched subgrid solves (mhx_batch_*): many unit-cube Q1 thermal problems
+    assembled and solved in one persistent launch.  Mirrors the role of
+    SubGridDtN_Solver::nonlinearSolver looped over macro groups
+    (src/subgrid/subgridDtN_solver.cpp:903-1043)."""
+
+    def __init__(self, n_axis, k, g, tol: float = 1e-10, precision: str = "fp64", wg_size: int = 0,
+                 use_graph: bool = True, max_iters: int = 10000, inner_tol: float = 1e-5, max_refine: int = 20):
+        self.n_axis = np.ascontiguousarray(n_axis, dtype=np.int32)
+        self._k = np.ascontiguousarray(k, dtype=np.float64)
+        self._g = np.ascontiguousarray(g, dtype=np.float64)
+        d = capi.BatchDesc()
+        d.n_inst = len(self.n_axis)
+        d.n_axis = self.n_axis.ctypes.data_as(C.POINTER(C.c_int32))
+        d.k = self._k.ctypes.data_as(C.POINTER(C.c_double))
+        d.g = self._g.ctypes.data_as(C.POINTER(C.c_double))
+        d.tol, d.max_iters = float(tol), int(max_iters)
+        d.precision = capi.MHX_BATCH_FP32_REFINED if precision == "fp32-refined" else capi.MHX_BATCH_FP64
+        d.inner_tol, d.max_refine = float(inner_tol), int(max_refine)
+        d.wg_size, d.use_graph = int(wg_size), int(use_graph)
+        h = C.c_void_p()
+        rc = lib.mhx_batch_create(C.byref(d), C.byref(h))
+        if rc != 0:
+            raise MhxError(rc, "mhx_batch_create: " + lib.mhx_last_error(None).decode())
+        self._h = h
+        nn, wg, ms = C.c_int64(), C.c_int32(), C.c_double()
+        lib.mhx_batch_info(h, C.byref(nn), C.byref(wg), C.byref(ms))
+        self.total_nodes, self.wg_size = nn.value, wg.value
+        self.offsets = np.concatenate([[0], np.cumsum((self.n_axis.astype(np.int64) + 1) ** 3)])
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib.mhx_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_wg(self, wg: int):
+        if lib.mhx_batch_set_wg(self._h, int(wg)) != 0:
+            raise MhxError(-1, "batch workgroup size must be 512 or 1024")
+        self.wg_size = wg
+
+    def solve(self) -> float:
+        """One batched assemble+solve of every instance; returns device milliseconds."""
+        rc = lib.mhx_batch_solve(self._h, 0)
+        if rc != 0:
+            raise MhxError(rc, "mhx_batch_solve")
+        nn, wg, ms = C.c_int64(), C.c_int32(), C.c_double()
+        lib.mhx_batch_info(self._h, C.byref(nn), C.byref(wg), C.byref(ms))
+        return ms.value
+
+    def results(self):
+        n = len(self.n_axis)
+        x = np.empty(self.total_nodes)
+        it = np.empty(n, dtype=np.int32)
+        rel = np.empty(n)
+        refs = np.empty(n, dtype=np.int32)
+        rc = lib.mhx_batch_results(self._h, x.ctypes.data, it.ctypes.data, rel.ctypes.data, refs.ctypes.data)
+        if rc != 0:
+            raise MhxError(rc, "mhx_batch_results")
+        return x, it, rel, refs
+
+    def instance(self, x, i):
+        return x[self.offsets[i]:self.offsets[i + 1]]
