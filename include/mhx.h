@@ -250,8 +250,9 @@ int  mhx_set_stream(mhx_ctx* ctx, void* hip_stream);
 
 /* ---- seeded inputs ------------------------------------------------------
  * Counter-based generator (splitmix64 of seed ^ index) so every consumer
- * produces identical fields.  kind 0: uniform [a,b); kind 1: exp(b*z) with
- * z ~ N(a?0:0,1) — lognormal with log-mean a, log-sd b. */
+ * produces identical fields.  With z ~ N(0,1) from Box-Muller of draws 2i
+ * and 2i+1: kind 0 uniform a + (b-a) u on [a,b); kind 1 lognormal
+ * exp(a + b z) (log-mean a, log-sd b); kind 2 normal a + b z. */
 #define MHX_RNG_UNIFORM   0
 #define MHX_RNG_LOGNORMAL 1
 #define MHX_RNG_NORMAL    2

@@ -8,8 +8,7 @@
 // It restates the reference algorithm in plain scalar C++ (FP64), following
 // the reference files cited at each function, with a Sacado-SFad-style dual
 // number.  Pinning (what checks each part against something other than a
-// retyped formula) is listed in oracle/PINNING.md and exercised by
-// tests/test_oracle.py:
+// retyped formula) is exercised by tests/test_oracle.py:
 //   basis/quadrature/geometry  -> regression/discretization/HGRAD/mrhyde.gold
 //   thermal weak form, DBCs    -> regression/thermal/*/mrhyde.gold L2 errors
 //   dual-number Jacobian       -> central finite differences of the residual
