@@ -4,12 +4,15 @@
 Workload (one GPU): [0,1]^3, 96^3 Q2 hexahedra (193^3 = 7,189,057 DOFs,
 454,756,609 CRS nonzeros), 3x3x3 Gauss points, k(T,x) = k0_e (1 + T^2) with
 k0_e = exp(0.5 z_e) (z_e ~ N(0,1), seed 2024), source f_e ~ U[0,1] (seed 2025),
-T = 0 on x = 0 and x = 1.  A step is one full J+R assembly (zero-fill, eight
-colour launches of the fused kernel, Dirichlet rows) with u resident in HBM.
+T = 0 on x = 0 and x = 1.  A step is one full J+R assembly (eight colour
+launches of the fused kernel; the first element to touch an entry stores it,
+Dirichlet rows included) with u, res and J resident in HBM.
 
 Prints one JSON line: assembly MDOF/s plus the roofline of the dominant kernel
 (algorithmic bytes B_asm per launch / its mean HIP-event duration) and a CPU
-baseline (the oracle, single thread, on a 24^3 sample scaled per element).
+baseline (the oracle, single thread, on a 24^3 sample scaled per element);
+beside them the GMRES iteration rate on the assembled Jacobian and the
+batched-subgrid solve rate.
 """
 from __future__ import annotations
 
@@ -69,6 +72,36 @@ def cpu_baseline(n_full: int, sample_n: int = 24):
     return {"value": n_full_dof / t_full / 1e6, "unit": "MDOF/s", "cores": 1, "kind": "port",
             "sample": f"oracle J+R assembly of the same problem on a {sample_n}^3 Q2 mesh ({ne_s} elements, "
                       f"{dt:.2f} s), scaled per element to {n_full}^3"}
+
+
+def krylov_bench(g, iters: int):
+    """GMRES(50) with Jacobi on the assembled Jacobian (device-resident), a
+    fixed number of iterations: all-FP64, and FP32 Jacobian / vectors / inner
+    GMRES inside FP64 refinement.  Per-iteration HIP-event time against the
+    algorithmic bytes of an average GMRES(m) iteration,
+    B_spmv + w N (m + 3) with w the vector width."""
+    from mrhyde_amd.abi import MHX_BUF_DU
+
+    N, nnz, m = g.n_dof, g.nnz, 50
+    out = {"workload": f"GMRES({m}) + Jacobi on the assembled Jacobian, {iters} iterations from du = 0"}
+    for name, prec, wv, wa in (("fp64", "jacobi", 8, 8), ("fp32-inner", "jacobi-fp32", 4, 4)):
+        g.state_upload(MHX_BUF_DU, np.zeros(N))
+        g.synchronize()
+        g.profile(True)
+        g.profile_reset()
+        t0 = time.perf_counter()
+        st = g.solve_resident(solver="gmres", restart=m, tol=1e-14, max_iters=iters, precond=prec)
+        g.synchronize()
+        dt = time.perf_counter() - t0
+        prof = g.profile_read()
+        g.profile(False)
+        it_ms, it_n = prof.get("Krylov::gmres_iteration", (float("nan"), 0))
+        per_it = it_ms / max(it_n, 1)
+        b_it = (wa + 4) * nnz + 4 * (N + 1) + 2 * wv * N + wv * N * (m + 3)
+        out[name] = {"iterations": st["iters"], "refinements": st["refinements"], "rel_res": st["rel_res"],
+                     "wall_s": round(dt, 3), "ms_per_iteration": round(per_it, 4),
+                     "bytes_per_iteration": b_it, "achieved_GBps": round(b_it / (per_it * 1e-3) / 1e9, 1)}
+    return out
 
 
 def subgrid_bench(steps: int, per_call_sample: int = 16):
@@ -142,6 +175,7 @@ def main():
     ap.add_argument("--colouring", type=int, default=0, help="0 element colouring, 1 block colouring")
     ap.add_argument("--probe-gib", type=float, default=1.0, help="size of the plain-copy bandwidth probe")
     ap.add_argument("--skip-subgrid", action="store_true", help="skip the batched-subgrid measurement")
+    ap.add_argument("--krylov-iters", type=int, default=100, help="GMRES iterations timed (0 = skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
@@ -206,6 +240,7 @@ def main():
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(n)
 
+    krylov = krylov_bench(g, args.krylov_iters) if args.krylov_iters > 0 else None
     subgrid = None if args.skip_subgrid else subgrid_bench(max(3, min(args.steps, 10)))
 
     out = {
@@ -232,6 +267,7 @@ def main():
                      "colouring": "block" if args.colouring else "element"},
         "cpu_baseline": cpu,
         "phases_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in prof.items()},
+        "krylov": krylov,
         "subgrid": subgrid,
     }
     print(json.dumps(out), flush=True)

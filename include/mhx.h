@@ -88,13 +88,19 @@ typedef struct mhx_problem_desc {
   double  dbc_const;      /* value when dbc_values == NULL */
   const double* dbc_values; /* [n_dof] nodal values (only Dirichlet entries read) or NULL */
 
-  /* Navier-Stokes (src/physics/navierstokes.cpp) */
-  double  ns_rho;         /* "density"   */
-  double  ns_mu;          /* "viscosity" */
-  int32_t ns_supg, ns_pspg;
-  uint32_t ns_lid_sides;  /* sides carrying the lid velocity ns_lid_u */
+  /* Navier-Stokes (src/physics/navierstokes.cpp), steady, Q1-Q1: unknowns
+   * per node (ux, uy, [uz], pr), DOF = node * (dim+1) + field.  Velocity
+   * component c is Dirichlet on dbc_sides | ns_lid_sides | ns_comp_sides[c]
+   * (the reference's per-variable "Dirichlet conditions" lists): ns_lid_u[c]
+   * on the lid sides (edges shared with other sides included), 0 elsewhere. */
+  double  ns_rho;         /* "density"   (> 0) */
+  double  ns_mu;          /* "viscosity" (> 0) */
+  int32_t ns_supg, ns_pspg;  /* "useSUPG", "usePSPG" */
+  uint32_t ns_lid_sides;
   double  ns_lid_u[3];
-  int32_t reserved[8];
+  double  ns_force[3];    /* "source ux", "source uy", "source uz" (constants) */
+  int32_t ns_pin_pressure;/* 1: pr = 0 at the corner node (xmin, ymin[, zmin]) */
+  uint32_t ns_comp_sides[3];
 } mhx_problem_desc;
 
 typedef struct mhx_ctx mhx_ctx;

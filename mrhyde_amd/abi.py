@@ -51,7 +51,8 @@ class ProblemDesc(C.Structure):
         ("dbc_sides", C.c_uint32), ("dbc_const", C.c_double), ("dbc_values", _dp),
         ("ns_rho", C.c_double), ("ns_mu", C.c_double), ("ns_supg", C.c_int32), ("ns_pspg", C.c_int32),
         ("ns_lid_sides", C.c_uint32), ("ns_lid_u", C.c_double * 3),
-        ("reserved", C.c_int32 * 8),
+        ("ns_force", C.c_double * 3), ("ns_pin_pressure", C.c_int32),
+        ("ns_comp_sides", C.c_uint32 * 3),
     ]
 
 
@@ -151,15 +152,33 @@ class Problem:
     perturb_seed: int = 0
     workset_size: int = 100
     physics: int = MHX_PHYS_THERMAL
+    # Navier-Stokes ("density", "viscosity", "useSUPG", "usePSPG", lid, body force)
+    rho: float = 1.0
+    mu: float = 1.0
+    supg: bool = False
+    pspg: bool = False
+    lid_sides: int = 0
+    lid_u: Sequence[float] = (0.0, 0.0, 0.0)
+    force: Sequence[float] = (0.0, 0.0, 0.0)
+    pin_pressure: bool = False
+    comp_sides: Sequence[int] = (0, 0, 0)  # extra Dirichlet sides per velocity component
     _keep: list = field(default_factory=list, repr=False)
+
+    @property
+    def n_field(self) -> int:
+        return self.dim + 1 if self.physics == MHX_PHYS_NAVIER_STOKES else 1
 
     @property
     def n_elem(self) -> int:
         return int(np.prod(list(self.n)[: self.dim]))
 
     @property
-    def n_dof(self) -> int:
+    def n_node(self) -> int:
         return int(np.prod([self.order * int(x) + 1 for x in list(self.n)[: self.dim]]))
+
+    @property
+    def n_dof(self) -> int:
+        return self.n_node * self.n_field
 
     def desc(self) -> ProblemDesc:
         d = ProblemDesc()
@@ -196,5 +215,13 @@ class Problem:
                 d.mode_freq[m][a] = float(fr[a])
         d.dbc_sides, d.dbc_const = int(self.dbc_sides), float(self.dbc_const)
         d.dbc_values = arr(self.dbc_values, self.n_dof)
+        d.ns_rho, d.ns_mu = float(self.rho), float(self.mu)
+        d.ns_supg, d.ns_pspg = int(bool(self.supg)), int(bool(self.pspg))
+        d.ns_lid_sides = int(self.lid_sides)
+        for a in range(3):
+            d.ns_lid_u[a] = float((list(self.lid_u) + [0.0] * 3)[a])
+            d.ns_force[a] = float((list(self.force) + [0.0] * 3)[a])
+            d.ns_comp_sides[a] = int((list(self.comp_sides) + [0] * 3)[a])
+        d.ns_pin_pressure = int(bool(self.pin_pressure))
         return d
 

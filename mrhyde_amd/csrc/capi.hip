@@ -69,9 +69,22 @@ static void validate(const mhx_problem_desc* d) {
   MHX_REQUIRE(d->xmax > d->xmin && d->ymax > d->ymin && (d->dim == 2 || d->zmax > d->zmin), MHX_ERR_ARG,
               "empty box");
   MHX_REQUIRE(d->n_modes >= 0 && d->n_modes <= MHX_MAX_MODES, MHX_ERR_ARG, "n_modes out of range");
+  if (d->physics == MHX_PHYS_NAVIER_STOKES) {
+    MHX_REQUIRE(d->ns_rho > 0.0 && d->ns_mu > 0.0, MHX_ERR_ARG, "navier stokes: density and viscosity must be > 0");
+    MHX_REQUIRE(ns_supported(d->dim, d->order, d->quadrature / 2 + 1), MHX_ERR_UNSUPPORTED,
+                "navier stokes: Q1-Q1 with quadrature 2 or 3 only");
+    return;
+  }
   MHX_REQUIRE(d->physics == MHX_PHYS_THERMAL, MHX_ERR_UNSUPPORTED, "physics module not built in this library");
   MHX_REQUIRE(thermal_supported(d->dim, d->order, d->quadrature / 2 + 1), MHX_ERR_UNSUPPORTED,
               "no assembly kernel for this dim/order/quadrature");
+}
+
+// physics dispatch (the reference's per-module volumeResidual, physicsImporter.cpp:76-272)
+static void assemble_physics(Ctx& c, const double* u, double* res, double* jac, bool want_res, bool want_jac,
+                             bool accumulate) {
+  if (c.desc.physics == MHX_PHYS_NAVIER_STOKES) assemble_ns(c, u, res, jac, want_res, want_jac, accumulate);
+  else assemble_thermal(c, u, res, jac, want_res, want_jac, accumulate);
 }
 
 // copy-in / copy-out of caller vectors for host-pointer calls
@@ -236,7 +249,7 @@ static int newton(Ctx& c, double* u, const mhx_newton_opts& o, mhx_solve_stats* 
   const int maxit = o.max_iters > 0 ? o.max_iters : 10;
   const double tol = o.nl_tol > 0 ? o.nl_tol : 1e-6;
   while (proceed) {
-    assemble_thermal(c, u, c.res.p, nullptr, true, false, false);  // assembleRes (ScalarT)
+    assemble_physics(c, u, c.res.p, nullptr, true, false, false);  // assembleRes (ScalarT)
     const double nrm = norm_inf(c, c.res.p, n);
     if (!std::isfinite(nrm)) throw Error{MHX_ERR_NONFINITE, "Newton: non-finite residual norm"};
     double scaled;
@@ -260,7 +273,7 @@ static int newton(Ctx& c, double* u, const mhx_newton_opts& o, mhx_solve_stats* 
       proceed = false;
     }
     if (solve) {
-      assemble_thermal(c, u, c.aux.p, c.jac.p, true, true, false);  // assembleJacRes (AD)
+      assemble_physics(c, u, c.aux.p, c.jac.p, true, true, false);  // assembleJacRes (AD)
       c.jac_valid = true;
       MHX_HIP(hipMemsetAsync(du.p, 0, n * sizeof(double), c.stream));
       int s = krylov_solve(c, c.jac.p, c.res.p, du.p, o.linear, st);
@@ -391,7 +404,8 @@ int mhx_dof_coords(const mhx_ctx* ctx, double* xyz) {
   if (!ctx || !xyz) return MHX_ERR_ARG;
   const BoxMesh& m = ctx->mesh;
   for (long g = 0; g < m.ndof; ++g) {
-    long c[3] = {g % m.nn[0], (g / m.nn[0]) % m.nn[1], g / ((long)m.nn[0] * m.nn[1])};
+    const long node = g / m.nf;
+    long c[3] = {node % m.nn[0], (node / m.nn[0]) % m.nn[1], node / ((long)m.nn[0] * m.nn[1])};
     long ec[3] = {0, 0, 0};
     double xi[3] = {0, 0, 0};
     for (int a = 0; a < m.dim; ++a) {
@@ -423,7 +437,7 @@ int mhx_assemble(mhx_ctx* ctx, const double* u, int32_t flags, double* res, doub
   VecArg U(*ctx, u, n, dev, ctx->stage_u, true);
   VecArg Rv(*ctx, res, want_res ? n : 0, dev, ctx->stage_r, acc);
   VecArg Jv(*ctx, jac_vals, want_jac ? ctx->nnz : 0, dev, ctx->stage_j, acc);
-  assemble_thermal(*ctx, U.dev, want_res ? Rv.dev : nullptr, want_jac ? Jv.dev : nullptr, want_res, want_jac, acc);
+  assemble_physics(*ctx, U.dev, want_res ? Rv.dev : nullptr, want_jac ? Jv.dev : nullptr, want_res, want_jac, acc);
   if (want_res) Rv.copy_out(res);
   if (want_jac) Jv.copy_out(jac_vals);
   if (!dev) MHX_HIP(hipStreamSynchronize(ctx->stream));
@@ -510,6 +524,7 @@ int mhx_l2_error(mhx_ctx* ctx, const double* u, int32_t n_modes, const double* a
   MHX_GUARD_BEGIN
   MHX_REQUIRE(ctx && u && l2 && l2_grad, MHX_ERR_ARG, "null argument");
   MHX_REQUIRE(n_modes >= 0 && n_modes <= MHX_MAX_MODES, MHX_ERR_ARG, "n_modes out of range");
+  MHX_REQUIRE(ctx->mesh.nf == 1, MHX_ERR_UNSUPPORTED, "L2 error of a single-field (thermal) solution only");
   const bool dev = (flags & MHX_DEVICE_PTRS) != 0;
   VecArg U(*ctx, u, ctx->mesh.ndof, dev, ctx->stage_a, true);
   ErrArgs a{};

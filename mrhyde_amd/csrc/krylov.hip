@@ -1,11 +1,17 @@
-// krylov.hip — CRS SpMV, wavefront-reduced dot products and the Jacobi-
-// preconditioned CG / restarted GMRES(m) drivers (FP64, optional FP32 inner
-// solve with FP64 iterative refinement).
+// krylov.hip — CRS SpMV, wavefront-reduced dot products, Jacobi / nodal
+// block-Jacobi preconditioners and the CG / restarted GMRES(m) drivers, in
+// FP64 or as an FP32 inner solve inside FP64 iterative refinement.
 //
 // Replaces LinearAlgebraInterface::linearSolver (Belos Block GMRES / CG with
 // Ifpack2 RELAXATION Jacobi; linearAlgebraInterface_solvers.hpp:14-152) and
 // SolverManager::PCG (solverManager_solvers.hpp:812-905).
-// Stopping rule for both: ||b - A x||_2 / ||b||_2 <= tol, FP64.
+// Stopping rule: ||b - A x||_2 / ||b||_2 <= tol, evaluated in FP64.
+//
+// Precision split of the mixed solve: the Jacobian values and the Krylov
+// vectors of the inner solve are FP32; every inner product (dots, norms,
+// Gram-Schmidt coefficients) accumulates in FP64, the Hessenberg / Givens
+// work is FP64 on the host, and the outer residual b - A x, the correction
+// x += ||r|| d and the stopping test are FP64 with the FP64 Jacobian.
 
 #include <algorithm>
 #include <cmath>
@@ -16,6 +22,11 @@
 namespace mhx {
 
 constexpr int kRedBlocks = 1024;  // partial sums per reduction
+constexpr int kScalH = 0;         // device scalars: Gram-Schmidt coefficients
+constexpr int kScalY = 96;        //                 GMRES least-squares solution
+constexpr int kScalN = 200;       //                 single reductions
+constexpr int kScalSize = 256;
+constexpr int kMaxRefine = 100;
 
 // y = A (s .* x)  (s == nullptr: plain SpMV).  LANES lanes per row.
 template <int LANES, class TA, class TX>
@@ -61,45 +72,39 @@ void spmv(Ctx& c, const double* A, const double* x, double* y) {
   spmv_t<double, double>(c, A, nullptr, x, y);
 }
 
-// ---- reductions: per-block partials in fixed order (deterministic) ----------
+// ---- reductions: per-block partials in fixed order (deterministic), FP64 ----
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
   return v;
 }
 
-template <int K>
-__device__ __forceinline__ void block_store(double (&v)[K], double* out, int stride) {
-  __shared__ double sh[K][4];
+__device__ __forceinline__ void block_store1(double v, double* out) {
+  __shared__ double sh[4];
   const int w = threadIdx.x / 64, l = threadIdx.x % 64;
-#pragma unroll
-  for (int k = 0; k < K; ++k) {
-    double s = wave_sum(v[k]);
-    if (l == 0) sh[k][w] = s;
-  }
+  const double s = wave_sum(v);
+  if (l == 0) sh[w] = s;
   __syncthreads();
-  if (threadIdx.x < K) {
-    const int k = threadIdx.x;
-    out[(long)k * stride + blockIdx.x] = sh[k][0] + sh[k][1] + sh[k][2] + sh[k][3];
-  }
+  if (threadIdx.x == 0) out[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
 }
 
 // max that propagates NaN (fmax would drop it and hide a non-finite residual)
 __device__ __forceinline__ double nanmax(double v, double x) { return (v != v || x <= v) ? v : x; }
 
-// mode 0: sum a*b ; mode 1: max |a| (NaN-propagating)
-__global__ __launch_bounds__(256) void k_dot(long n, const double* __restrict__ a, const double* __restrict__ b,
+// mode 0: sum a*b ; mode 1: max |a| (NaN-propagating); FP64 accumulation
+template <class T>
+__global__ __launch_bounds__(256) void k_dot(long n, const T* __restrict__ a, const T* __restrict__ b,
                                              double* __restrict__ part, int mode) {
-  double v[1] = {0.0};
+  double v = 0.0;
   for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
-    if (mode == 0) v[0] = fma(a[i], b[i], v[0]);
-    else v[0] = nanmax(v[0], fabs(a[i]));
+    if (mode == 0) v = fma((double)a[i], (double)b[i], v);
+    else v = nanmax(v, fabs((double)a[i]));
   }
   if (mode == 0) {
-    block_store<1>(v, part, kRedBlocks);
+    block_store1(v, part);
   } else {
     __shared__ double sh[256];
-    sh[threadIdx.x] = v[0];
+    sh[threadIdx.x] = v;
     __syncthreads();
     for (int s = 128; s > 0; s >>= 1) {
       if (threadIdx.x < s) sh[threadIdx.x] = nanmax(sh[threadIdx.x], sh[threadIdx.x + s]);
@@ -109,17 +114,17 @@ __global__ __launch_bounds__(256) void k_dot(long n, const double* __restrict__ 
   }
 }
 
-// reduce K rows of kRedBlocks partials (row stride kRedBlocks) into out[K]
+// reduce K rows of nb partials (row stride kRedBlocks) into out[K]
 __global__ __launch_bounds__(256) void k_final(const double* __restrict__ part, int nb, int K, int mode,
                                                double* __restrict__ out) {
+  __shared__ double sh[256];
   for (int k = 0; k < K; ++k) {
-    double v[1] = {0.0};
+    double v = 0.0;
     for (int i = threadIdx.x; i < nb; i += 256) {
       double p = part[(long)k * kRedBlocks + i];
-      v[0] = mode == 0 ? v[0] + p : nanmax(v[0], p);
+      v = mode == 0 ? v + p : nanmax(v, p);
     }
-    __shared__ double sh[256];
-    sh[threadIdx.x] = v[0];
+    sh[threadIdx.x] = v;
     __syncthreads();
     for (int s = 128; s > 0; s >>= 1) {
       if (threadIdx.x < s) sh[threadIdx.x] = mode == 0 ? sh[threadIdx.x] + sh[threadIdx.x + s] : nanmax(sh[threadIdx.x], sh[threadIdx.x + s]);
@@ -137,42 +142,77 @@ static int red_blocks(long n) {
 
 static void ensure_red(Ctx& c, int K) {
   if (c.kw.part.n < (size_t)K * kRedBlocks) c.kw.part.alloc((size_t)K * kRedBlocks);
-  if (c.kw.scal.n < (size_t)std::max(K + 160, 256)) c.kw.scal.alloc(std::max(K + 160, 256));
+  if (c.kw.scal.n < (size_t)kScalSize) c.kw.scal.alloc(kScalSize);
 }
 
-static double reduce_to_host(Ctx& c, long n, const double* a, const double* b, int mode) {
-  ensure_red(c, 1);
-  int nb = red_blocks(n);
-  hipLaunchKernelGGL(k_dot, dim3(nb), dim3(256), 0, c.stream, n, a, b, c.kw.part.p, mode);
-  hipLaunchKernelGGL(k_final, dim3(1), dim3(256), 0, c.stream, c.kw.part.p, nb, 1, mode, c.kw.scal.p);
+static double fetch(Ctx& c, int nb, int mode = 0) {
+  hipLaunchKernelGGL(k_final, dim3(1), dim3(256), 0, c.stream, c.kw.part.p, nb, 1, mode, c.kw.scal.p + kScalN);
   MHX_HIP(hipGetLastError());
   double h;
-  MHX_HIP(hipMemcpyAsync(&h, c.kw.scal.p, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  MHX_HIP(hipMemcpyAsync(&h, c.kw.scal.p + kScalN, sizeof(double), hipMemcpyDeviceToHost, c.stream));
   MHX_HIP(hipStreamSynchronize(c.stream));
   return h;
 }
 
+template <class T>
+static double reduce_to_host(Ctx& c, long n, const T* a, const T* b, int mode) {
+  ensure_red(c, 1);
+  const int nb = red_blocks(n);
+  hipLaunchKernelGGL(k_dot<T>, dim3(nb), dim3(256), 0, c.stream, n, a, b, c.kw.part.p, mode);
+  MHX_HIP(hipGetLastError());
+  return fetch(c, nb, mode);
+}
+
+template <class T>
+static double norm2_t(Ctx& c, const T* a, long n) {
+  return std::sqrt(reduce_to_host<T>(c, n, a, a, 0));
+}
+
 double dot(Ctx& c, const double* a, const double* b, long n) {
   ScopedPhase ph(c.prof, "Krylov::dot", c.stream);
-  return reduce_to_host(c, n, a, b, 0);
+  return reduce_to_host<double>(c, n, a, b, 0);
 }
 double norm2(Ctx& c, const double* a, long n) { return std::sqrt(dot(c, a, a, n)); }
 double norm_inf(Ctx& c, const double* a, long n) {
   ScopedPhase ph(c.prof, "Krylov::normInf", c.stream);
-  return reduce_to_host(c, n, a, a, 1);
+  return reduce_to_host<double>(c, n, a, a, 1);
 }
 
 // ---- elementwise kernels ----------------------------------------------------
+static dim3 grid1(long n) { return dim3((unsigned)((n + 255) / 256)); }
+
 __global__ void k_axpy(long n, double a, const double* __restrict__ x, double* __restrict__ y) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) y[i] = fma(a, x[i], y[i]);
 }
 void axpy(Ctx& c, long n, double a, const double* x, double* y) {
-  hipLaunchKernelGGL(k_axpy, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream, n, a, x, y);
+  hipLaunchKernelGGL(k_axpy, grid1(n), dim3(256), 0, c.stream, n, a, x, y);
   MHX_HIP(hipGetLastError());
 }
 
-// minv[i] = 1 / A(i,i)
+// y = (TO)(s * x)
+template <class TI, class TO>
+__global__ void k_cast_scale(long n, double s, const TI* __restrict__ x, TO* __restrict__ y) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = (TO)(s * (double)x[i]);
+}
+// y += a * x  (FP64 y, FP32 x: the refinement correction)
+__global__ void k_axpy_f2d(long n, double a, const float* __restrict__ x, double* __restrict__ y) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = fma(a, (double)x[i], y[i]);
+}
+template <class T>
+__global__ void k_add(long n, const T* __restrict__ a, T* __restrict__ x) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] += a[i];
+}
+template <class T>
+__global__ void k_mul(long n, const T* __restrict__ d, const T* __restrict__ v, T* __restrict__ out) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = d[i] * v[i];
+}
+
+// minv[i] = 1 / A(i,i)  (Ifpack2 RELAXATION "Jacobi")
 template <class T>
 __global__ void k_jacobi(long n, const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
                          const double* __restrict__ A, T* __restrict__ minv) {
@@ -187,6 +227,90 @@ __global__ void k_jacobi(long n, const int32_t* __restrict__ rp, const int32_t* 
   minv[i] = (T)(1.0 / A[lo]);
 }
 
+// Nodal block Jacobi: the BS x BS diagonal block of each node (its BS
+// consecutive DOFs, whose columns are consecutive in every sorted row of
+// the node), inverted in FP64 by Gauss-Jordan with partial pivoting.
+// A zero pivot leaves Inf/NaN, which the solver's residual check reports.
+template <int BS, class T>
+__global__ void k_block_inv(long nblk, const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
+                            const double* __restrict__ A, T* __restrict__ out) {
+  long blk = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (blk >= nblk) return;
+  double M[BS][BS], I[BS][BS];
+  const int c0 = (int)(blk * BS);
+#pragma unroll
+  for (int i = 0; i < BS; ++i) {
+    const int row = c0 + i;
+    int lo = rp[row], hi = rp[row + 1];
+    const int end = hi;
+    while (lo < hi) {
+      int mid = (lo + hi) >> 1;
+      if (ci[mid] < c0) lo = mid + 1;
+      else hi = mid;
+    }
+#pragma unroll
+    for (int j = 0; j < BS; ++j) {
+      const int k = lo + j;
+      M[i][j] = (k < end && ci[k] == c0 + j) ? A[k] : 0.0;
+      I[i][j] = i == j ? 1.0 : 0.0;
+    }
+  }
+#pragma unroll
+  for (int col = 0; col < BS; ++col) {
+    int piv = col;
+    double best = fabs(M[col][col]);
+#pragma unroll
+    for (int r = col + 1; r < BS; ++r)
+      if (fabs(M[r][col]) > best) {
+        best = fabs(M[r][col]);
+        piv = r;
+      }
+#pragma unroll
+    for (int r = col + 1; r < BS; ++r)
+      if (r == piv)
+#pragma unroll
+        for (int j = 0; j < BS; ++j) {
+          double t = M[col][j]; M[col][j] = M[r][j]; M[r][j] = t;
+          t = I[col][j]; I[col][j] = I[r][j]; I[r][j] = t;
+        }
+    const double d = 1.0 / M[col][col];
+#pragma unroll
+    for (int j = 0; j < BS; ++j) {
+      M[col][j] *= d;
+      I[col][j] *= d;
+    }
+#pragma unroll
+    for (int r = 0; r < BS; ++r) {
+      if (r == col) continue;
+      const double f = M[r][col];
+#pragma unroll
+      for (int j = 0; j < BS; ++j) {
+        M[r][j] = fma(-f, M[col][j], M[r][j]);
+        I[r][j] = fma(-f, I[col][j], I[r][j]);
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < BS; ++i)
+#pragma unroll
+    for (int j = 0; j < BS; ++j) out[blk * BS * BS + i * BS + j] = (T)I[i][j];
+}
+
+// out = blockdiag(binv) v, one DOF per thread
+template <int BS, class T>
+__global__ void k_block_apply(long n, const T* __restrict__ binv, const T* __restrict__ v, T* __restrict__ out) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const long blk = i / BS;
+  const int r = (int)(i - blk * BS);
+  const T* row = binv + blk * BS * BS + r * BS;
+  const T* vb = v + blk * BS;
+  T s = 0;
+#pragma unroll
+  for (int j = 0; j < BS; ++j) s += row[j] * vb[j];
+  out[i] = s;
+}
+
 __global__ void k_set_dbc(long n, const int32_t* __restrict__ list, const double* __restrict__ vals,
                           double* __restrict__ u) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -195,111 +319,188 @@ __global__ void k_set_dbc(long n, const int32_t* __restrict__ list, const double
 void set_dirichlet_dev(Ctx& c, double* u) {
   long n = (long)c.h_dbc_list.size();
   if (!n) return;
-  hipLaunchKernelGGL(k_set_dbc, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream, n, c.dbc_list.p, c.dbc_vals.p, u);
+  hipLaunchKernelGGL(k_set_dbc, grid1(n), dim3(256), 0, c.stream, n, c.dbc_list.p, c.dbc_vals.p, u);
   MHX_HIP(hipGetLastError());
 }
 
-// r = b - A x ; returns ||r||^2 partials path via dot afterwards
-__global__ void k_resid(long n, const double* __restrict__ b, const double* __restrict__ Ax, double* __restrict__ r) {
+// r = b - Ax
+template <class T>
+__global__ void k_resid(long n, const T* __restrict__ b, const T* __restrict__ Ax, T* __restrict__ r) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) r[i] = b[i] - Ax[i];
 }
 
-// ---- CG (Jacobi PCG as SolverManager::PCG) -----------------------------------
-// z = minv .* r  fused with rho = r.z
-__global__ __launch_bounds__(256) void k_prec_dot(long n, const double* __restrict__ minv, const double* __restrict__ r,
-                                                  double* __restrict__ z, double* __restrict__ part) {
-  double v[1] = {0.0};
-  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
-    double zi = minv[i] * r[i];
-    z[i] = zi;
-    v[0] = fma(r[i], zi, v[0]);
+template <class T>
+static KrylovVecs<T>& vecs(Ctx& c);
+template <>
+KrylovVecs<double>& vecs<double>(Ctx& c) { return c.kw.d; }
+template <>
+KrylovVecs<float>& vecs<float>(Ctx& c) { return c.kw.f; }
+
+// ---- preconditioners -------------------------------------------------------
+template <class T>
+struct Prec {
+  int kind = MHX_PREC_NONE;
+  int bs = 1;
+  const T* d = nullptr;  // 1/diag (point) or BS x BS inverses (block)
+};
+
+template <class T>
+static Prec<T> build_prec(Ctx& c, const double* A, int kind, KrylovVecs<T>& v) {
+  const long n = c.mesh.ndof;
+  Prec<T> P;
+  if (kind == MHX_PREC_BLOCK_JACOBI && c.nfield == 1) kind = MHX_PREC_JACOBI;
+  P.kind = kind;
+  if (kind == MHX_PREC_JACOBI) {
+    v.prec.alloc(n);
+    hipLaunchKernelGGL(k_jacobi<T>, grid1(n), dim3(256), 0, c.stream, n, c.rowptr.p, c.colidx.p, A, v.prec.p);
+  } else if (kind == MHX_PREC_BLOCK_JACOBI) {
+    const int bs = c.nfield;
+    const long nblk = n / bs;
+    v.prec.alloc((size_t)nblk * bs * bs);
+    P.bs = bs;
+    switch (bs) {
+      case 2: hipLaunchKernelGGL((k_block_inv<2, T>), grid1(nblk), dim3(256), 0, c.stream, nblk, c.rowptr.p, c.colidx.p, A, v.prec.p); break;
+      case 3: hipLaunchKernelGGL((k_block_inv<3, T>), grid1(nblk), dim3(256), 0, c.stream, nblk, c.rowptr.p, c.colidx.p, A, v.prec.p); break;
+      case 4: hipLaunchKernelGGL((k_block_inv<4, T>), grid1(nblk), dim3(256), 0, c.stream, nblk, c.rowptr.p, c.colidx.p, A, v.prec.p); break;
+      default: throw Error{MHX_ERR_UNSUPPORTED, "block Jacobi: block size must be 1..4"};
+    }
   }
-  block_store<1>(v, part, kRedBlocks);
+  MHX_HIP(hipGetLastError());
+  P.d = v.prec.p;
+  return P;
 }
-// p = z + beta p
-__global__ void k_xpby(long n, const double* __restrict__ z, double beta, double* __restrict__ p) {
+
+// out = M^-1 v  (out != v)
+template <class T>
+static void prec_apply(Ctx& c, const Prec<T>& P, const T* v, T* out) {
+  const long n = c.mesh.ndof;
+  if (P.kind == MHX_PREC_JACOBI) {
+    hipLaunchKernelGGL(k_mul<T>, grid1(n), dim3(256), 0, c.stream, n, P.d, v, out);
+  } else if (P.kind == MHX_PREC_BLOCK_JACOBI) {
+    switch (P.bs) {
+      case 2: hipLaunchKernelGGL((k_block_apply<2, T>), grid1(n), dim3(256), 0, c.stream, n, P.d, v, out); break;
+      case 3: hipLaunchKernelGGL((k_block_apply<3, T>), grid1(n), dim3(256), 0, c.stream, n, P.d, v, out); break;
+      default: hipLaunchKernelGGL((k_block_apply<4, T>), grid1(n), dim3(256), 0, c.stream, n, P.d, v, out); break;
+    }
+  } else {
+    MHX_HIP(hipMemcpyAsync(out, v, n * sizeof(T), hipMemcpyDeviceToDevice, c.stream));
+  }
+  MHX_HIP(hipGetLastError());
+}
+
+// w = A M^-1 v (the point-Jacobi scaling is fused into the SpMV)
+template <class T>
+static void op_apply(Ctx& c, const T* A, const Prec<T>& P, const T* v, T* w, T* tmp) {
+  if (P.kind == MHX_PREC_JACOBI) {
+    spmv_t<T, T>(c, A, P.d, v, w);
+  } else if (P.kind == MHX_PREC_BLOCK_JACOBI) {
+    prec_apply<T>(c, P, v, tmp);
+    spmv_t<T, T>(c, A, nullptr, tmp, w);
+  } else {
+    spmv_t<T, T>(c, A, nullptr, v, w);
+  }
+}
+
+// ---- CG (Jacobi PCG as SolverManager::PCG) -----------------------------------
+// z = minv .* r (minv == nullptr: z = r) fused with rho = r.z
+template <class T>
+__global__ __launch_bounds__(256) void k_prec_dot(long n, const T* __restrict__ minv, const T* __restrict__ r,
+                                                  T* __restrict__ z, double* __restrict__ part) {
+  double v = 0.0;
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+    const T zi = minv ? minv[i] * r[i] : r[i];
+    z[i] = zi;
+    v = fma((double)r[i], (double)zi, v);
+  }
+  block_store1(v, part);
+}
+// p = z + beta p ; first iteration p = z (p may hold uninitialised bits,
+// and 0 * NaN would not clear them)
+template <class T>
+__global__ void k_xpby(long n, const T* __restrict__ z, double beta, int first, T* __restrict__ p) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = fma(beta, p[i], z[i]);
+  if (i < n) p[i] = first ? z[i] : (T)fma(beta, (double)p[i], (double)z[i]);
 }
 // x += alpha p ; r -= alpha q ; partial ||r||^2
-__global__ __launch_bounds__(256) void k_cg_update(long n, double alpha, const double* __restrict__ p,
-                                                   const double* __restrict__ q, double* __restrict__ x,
-                                                   double* __restrict__ r, double* __restrict__ part) {
-  double v[1] = {0.0};
+template <class T>
+__global__ __launch_bounds__(256) void k_cg_update(long n, double alpha, const T* __restrict__ p,
+                                                   const T* __restrict__ q, T* __restrict__ x, T* __restrict__ r,
+                                                   double* __restrict__ part) {
+  double v = 0.0;
+  const T a = (T)alpha;
   for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
-    x[i] = fma(alpha, p[i], x[i]);
-    double ri = fma(-alpha, q[i], r[i]);
+    x[i] = fma(a, p[i], x[i]);
+    const T ri = fma(-a, q[i], r[i]);
     r[i] = ri;
-    v[0] = fma(ri, ri, v[0]);
+    v = fma((double)ri, (double)ri, v);
   }
-  block_store<1>(v, part, kRedBlocks);
+  block_store1(v, part);
 }
 
-static double fetch_scalar(Ctx& c, int nb) {
-  hipLaunchKernelGGL(k_final, dim3(1), dim3(256), 0, c.stream, c.kw.part.p, nb, 1, 0, c.kw.scal.p);
-  double h;
-  MHX_HIP(hipMemcpyAsync(&h, c.kw.scal.p, sizeof(double), hipMemcpyDeviceToHost, c.stream));
-  MHX_HIP(hipStreamSynchronize(c.stream));
-  return h;
-}
-
-static int cg_solve(Ctx& c, const double* A, const double* b, double* x, double tol, int maxit, double* relres) {
+template <class T>
+static int cg_core(Ctx& c, const T* A, const Prec<T>& P, const T* b, T* x, double tol, int maxit, double* relres) {
   const long n = c.mesh.ndof;
-  KrylovWork& w = c.kw;
-  w.r.alloc(n); w.z.alloc(n); w.p.alloc(n); w.q.alloc(n); w.minv.alloc(n);
+  KrylovVecs<T>& v = vecs<T>(c);
+  v.r.alloc(n); v.z.alloc(n); v.p.alloc(n); v.q.alloc(n);
   ensure_red(c, 1);
-  const dim3 g1((unsigned)((n + 255) / 256));
   const int nb = red_blocks(n);
-  hipLaunchKernelGGL(k_jacobi<double>, g1, dim3(256), 0, c.stream, n, c.rowptr.p, c.colidx.p, A, w.minv.p);
-  spmv(c, A, x, w.q.p);
-  hipLaunchKernelGGL(k_resid, g1, dim3(256), 0, c.stream, n, b, w.q.p, w.r.p);
-  const double bn = norm2(c, b, n);
-  double rn = norm2(c, w.r.p, n);
+  const double bn = norm2_t<T>(c, b, n);
   if (bn == 0.0) {
-    MHX_HIP(hipMemsetAsync(x, 0, n * sizeof(double), c.stream));
+    MHX_HIP(hipMemsetAsync(x, 0, n * sizeof(T), c.stream));
     *relres = 0.0;
     return 0;
   }
+  spmv_t<T, T>(c, A, nullptr, x, v.q.p);
+  hipLaunchKernelGGL(k_resid<T>, grid1(n), dim3(256), 0, c.stream, n, b, v.q.p, v.r.p);
+  double rn = norm2_t<T>(c, v.r.p, n);
   double rho = 1.0;
   int it = 0;
   while (it < maxit && rn / bn > tol) {
     ScopedPhase ph(c.prof, "Krylov::cg_iteration", c.stream);
-    hipLaunchKernelGGL(k_prec_dot, dim3(nb), dim3(256), 0, c.stream, n, w.minv.p, w.r.p, w.z.p, w.part.p);
     const double rho1 = rho;
-    rho = fetch_scalar(c, nb);
+    if (P.kind == MHX_PREC_BLOCK_JACOBI) {
+      prec_apply<T>(c, P, v.r.p, v.z.p);
+      rho = reduce_to_host<T>(c, n, v.r.p, v.z.p, 0);
+    } else {
+      hipLaunchKernelGGL(k_prec_dot<T>, dim3(nb), dim3(256), 0, c.stream, n,
+                         P.kind == MHX_PREC_JACOBI ? P.d : (const T*)nullptr, v.r.p, v.z.p, c.kw.part.p);
+      rho = fetch(c, nb);
+    }
     const double beta = it == 0 ? 0.0 : rho / rho1;
-    hipLaunchKernelGGL(k_xpby, g1, dim3(256), 0, c.stream, n, w.z.p, beta, w.p.p);
-    spmv_t<double, double>(c, A, nullptr, w.p.p, w.q.p);
-    const double pq = reduce_to_host(c, n, w.p.p, w.q.p, 0);
+    hipLaunchKernelGGL(k_xpby<T>, grid1(n), dim3(256), 0, c.stream, n, v.z.p, beta, it == 0 ? 1 : 0, v.p.p);
+    spmv_t<T, T>(c, A, nullptr, v.p.p, v.q.p);
+    const double pq = reduce_to_host<T>(c, n, v.p.p, v.q.p, 0);
     const double alpha = rho / pq;
-    hipLaunchKernelGGL(k_cg_update, dim3(nb), dim3(256), 0, c.stream, n, alpha, w.p.p, w.q.p, x, w.r.p, w.part.p);
-    rn = std::sqrt(fetch_scalar(c, nb));
+    hipLaunchKernelGGL(k_cg_update<T>, dim3(nb), dim3(256), 0, c.stream, n, alpha, v.p.p, v.q.p, x, v.r.p,
+                       c.kw.part.p);
+    rn = std::sqrt(fetch(c, nb));
     ++it;
     if (!std::isfinite(rn)) throw Error{MHX_ERR_NONFINITE, "CG: non-finite residual"};
   }
   // true residual
-  spmv(c, A, x, w.q.p);
-  hipLaunchKernelGGL(k_resid, g1, dim3(256), 0, c.stream, n, b, w.q.p, w.r.p);
-  *relres = norm2(c, w.r.p, n) / bn;
+  spmv_t<T, T>(c, A, nullptr, x, v.q.p);
+  hipLaunchKernelGGL(k_resid<T>, grid1(n), dim3(256), 0, c.stream, n, b, v.q.p, v.r.p);
+  *relres = norm2_t<T>(c, v.r.p, n) / bn;
   return it;
 }
 
-// ---- GMRES(m), right Jacobi preconditioning, CGS + DGKS re-orthogonalisation --
-// h[k] = V_k . w for k < K  (V row-major: vector k at V + k*ld)
-__global__ __launch_bounds__(256) void k_mdot(long n, int K, const double* __restrict__ V, long ld,
-                                              const double* __restrict__ w, double* __restrict__ part) {
-  // K <= 64 handled in chunks of 8 to bound registers
+// ---- GMRES(m), right preconditioning, CGS + DGKS re-orthogonalisation --------
+// h[k] = V_k . w for k < K  (V row-major: vector k at V + k*ld), FP64 sums
+template <class T>
+__global__ __launch_bounds__(256) void k_mdot(long n, int K, const T* __restrict__ V, long ld,
+                                              const T* __restrict__ w, double* __restrict__ part) {
+  // K <= 65 handled in chunks of 8 to bound registers
+  __shared__ double sh[8][4];
   for (int k0 = 0; k0 < K; k0 += 8) {
     double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     const int kk = min(8, K - k0);
     for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
-      const double wi = w[i];
+      const double wi = (double)w[i];
 #pragma unroll
       for (int k = 0; k < 8; ++k)
-        if (k < kk) v[k] = fma(V[(long)(k0 + k) * ld + i], wi, v[k]);
+        if (k < kk) v[k] = fma((double)V[(long)(k0 + k) * ld + i], wi, v[k]);
     }
-    __shared__ double sh[8][4];
     const int wv = threadIdx.x / 64, l = threadIdx.x % 64;
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
@@ -312,87 +513,87 @@ __global__ __launch_bounds__(256) void k_mdot(long n, int K, const double* __res
     __syncthreads();
   }
 }
-// w -= sum_k h[k] V_k ; partial ||w||^2
-__global__ __launch_bounds__(256) void k_mupdate(long n, int K, const double* __restrict__ V, long ld,
-                                                 const double* __restrict__ h, double* __restrict__ w,
+// w -= sum_k h[k] V_k (FP64 combination) ; partial ||w||^2
+template <class T>
+__global__ __launch_bounds__(256) void k_mupdate(long n, int K, const T* __restrict__ V, long ld,
+                                                 const double* __restrict__ h, T* __restrict__ w,
                                                  double* __restrict__ part) {
-  double v[1] = {0.0};
+  double v = 0.0;
   for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
-    double wi = w[i];
-    for (int k = 0; k < K; ++k) wi = fma(-h[k], V[(long)k * ld + i], wi);
-    w[i] = wi;
-    v[0] = fma(wi, wi, v[0]);
+    double wi = (double)w[i];
+    for (int k = 0; k < K; ++k) wi = fma(-h[k], (double)V[(long)k * ld + i], wi);
+    const T ws = (T)wi;
+    w[i] = ws;
+    v = fma((double)ws, (double)ws, v);
   }
-  block_store<1>(v, part, kRedBlocks);
+  block_store1(v, part);
 }
-__global__ void k_scale_to(long n, double s, const double* __restrict__ x, double* __restrict__ y) {
+template <class T>
+__global__ void k_scale_to(long n, double s, const T* __restrict__ x, T* __restrict__ y) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) y[i] = s * x[i];
+  if (i < n) y[i] = (T)(s * (double)x[i]);
 }
-// x += minv .* (V y)
-__global__ void k_gm_update(long n, int K, const double* __restrict__ V, long ld, const double* __restrict__ y,
-                            const double* __restrict__ minv, double* __restrict__ x) {
+// out = V y
+template <class T>
+__global__ void k_vcomb(long n, int K, const T* __restrict__ V, long ld, const double* __restrict__ y,
+                        T* __restrict__ out) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double s = 0.0;
-  for (int k = 0; k < K; ++k) s = fma(y[k], V[(long)k * ld + i], s);
-  x[i] = fma(minv[i], s, x[i]);
+  for (int k = 0; k < K; ++k) s = fma(y[k], (double)V[(long)k * ld + i], s);
+  out[i] = (T)s;
 }
 
-static int gmres_solve(Ctx& c, const double* A, const double* b, double* x, int m, double tol, int maxit,
-                       double* relres) {
+template <class T>
+static int gmres_core(Ctx& c, const T* A, const Prec<T>& P, const T* b, T* x, int m, double tol, int maxit,
+                      double* relres) {
   const long n = c.mesh.ndof;
-  KrylovWork& w = c.kw;
+  KrylovVecs<T>& v = vecs<T>(c);
   m = std::max(1, std::min(m, 64));
-  w.r.alloc(n); w.w.alloc(n); w.minv.alloc(n); w.q.alloc(n);
-  if (w.V.n < (size_t)(m + 1) * n) w.V.alloc((size_t)(m + 1) * n);
+  v.r.alloc(n); v.q.alloc(n); v.tmp.alloc(n); v.tmp2.alloc(n);
+  if (v.V.n < (size_t)(m + 1) * n) v.V.alloc((size_t)(m + 1) * n);
   ensure_red(c, m + 1);
-  const dim3 g1((unsigned)((n + 255) / 256));
   const int nb = red_blocks(n);
-  hipLaunchKernelGGL(k_jacobi<double>, g1, dim3(256), 0, c.stream, n, c.rowptr.p, c.colidx.p, A, w.minv.p);
-  const double bn = norm2(c, b, n);
+  const double bn = norm2_t<T>(c, b, n);
   if (bn == 0.0) {
-    MHX_HIP(hipMemsetAsync(x, 0, n * sizeof(double), c.stream));
+    MHX_HIP(hipMemsetAsync(x, 0, n * sizeof(T), c.stream));
     *relres = 0.0;
     return 0;
   }
-  std::vector<double> H((m + 1) * m), cs(m), sn(m), g(m + 1), hcol(m + 1), y(m);
+  std::vector<double> H((m + 1) * m), cs(m), sn(m), g(m + 1), hcol(m + 1), y(m), hh(m + 1);
+  double* dscal = c.kw.scal.p;
   int it = 0;
   double rel = 1.0;
   for (;;) {
-    spmv(c, A, x, w.q.p);
-    hipLaunchKernelGGL(k_resid, g1, dim3(256), 0, c.stream, n, b, w.q.p, w.r.p);
-    const double beta = norm2(c, w.r.p, n);
+    spmv_t<T, T>(c, A, nullptr, x, v.q.p);
+    hipLaunchKernelGGL(k_resid<T>, grid1(n), dim3(256), 0, c.stream, n, b, v.q.p, v.r.p);
+    const double beta = norm2_t<T>(c, v.r.p, n);
     rel = beta / bn;
     if (!std::isfinite(rel)) throw Error{MHX_ERR_NONFINITE, "GMRES: non-finite residual"};
     if (rel <= tol || it >= maxit) break;
-    hipLaunchKernelGGL(k_scale_to, g1, dim3(256), 0, c.stream, n, 1.0 / beta, w.r.p, w.V.p);
+    hipLaunchKernelGGL(k_scale_to<T>, grid1(n), dim3(256), 0, c.stream, n, 1.0 / beta, v.r.p, v.V.p);
     std::fill(g.begin(), g.end(), 0.0);
     g[0] = beta;
     int k = 0;
     for (; k < m && it < maxit;) {
       ScopedPhase ph(c.prof, "Krylov::gmres_iteration", c.stream);
-      double* vk = w.V.p + (long)k * n;
-      double* vk1 = w.V.p + (long)(k + 1) * n;
-      spmv_t<double, double>(c, A, w.minv.p, vk, vk1);  // w = A M^-1 v_k, stored in V_{k+1}
-      double wn0 = 0.0;
-      // CGS pass 1
+      T* vk = v.V.p + (long)k * n;
+      T* vk1 = v.V.p + (long)(k + 1) * n;
+      op_apply<T>(c, A, P, vk, vk1, v.tmp.p);  // w = A M^-1 v_k, stored in V_{k+1}
       std::fill(hcol.begin(), hcol.end(), 0.0);
+      double wn0 = 0.0;
       for (int pass = 0; pass < 2; ++pass) {
-        hipLaunchKernelGGL(k_mdot, dim3(nb), dim3(256), 0, c.stream, n, k + 1, w.V.p, n, vk1, w.part.p);
-        hipLaunchKernelGGL(k_final, dim3(1), dim3(256), 0, c.stream, w.part.p, nb, k + 1, 0, w.scal.p);
-        std::vector<double> hh(k + 1);
-        MHX_HIP(hipMemcpyAsync(hh.data(), w.scal.p, (k + 1) * sizeof(double), hipMemcpyDeviceToHost, c.stream));
-        MHX_HIP(hipStreamSynchronize(c.stream));
+        // h = V^T w on the device; w -= V h reads h there; one host sync per pass
+        hipLaunchKernelGGL(k_mdot<T>, dim3(nb), dim3(256), 0, c.stream, n, k + 1, v.V.p, n, vk1, c.kw.part.p);
+        hipLaunchKernelGGL(k_final, dim3(1), dim3(256), 0, c.stream, c.kw.part.p, nb, k + 1, 0, dscal + kScalH);
+        MHX_HIP(hipMemcpyAsync(hh.data(), dscal + kScalH, (k + 1) * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+        hipLaunchKernelGGL(k_mupdate<T>, dim3(nb), dim3(256), 0, c.stream, n, k + 1, v.V.p, n, dscal + kScalH, vk1,
+                           c.kw.part.p);
+        const double wn2 = fetch(c, nb);
         if (pass == 0) {
-          double s = 0.0;
-          for (int j = 0; j <= k; ++j) s += hh[j] * hh[j];
-          // ||w||^2 before projection = ||w_after||^2 + sum h^2 (Pythagoras), computed after update
-          wn0 = s;
+          // ||w||^2 before the projection = ||w_after||^2 + sum h^2
+          for (int j = 0; j <= k; ++j) wn0 += hh[j] * hh[j];
         }
-        MHX_HIP(hipMemcpyAsync(w.scal.p + 128, hh.data(), (k + 1) * sizeof(double), hipMemcpyHostToDevice, c.stream));
-        hipLaunchKernelGGL(k_mupdate, dim3(nb), dim3(256), 0, c.stream, n, k + 1, w.V.p, n, w.scal.p + 128, vk1, w.part.p);
-        const double wn2 = fetch_scalar(c, nb);
         for (int j = 0; j <= k; ++j) hcol[j] += hh[j];
         hcol[k + 1] = std::sqrt(wn2);
         // DGKS criterion: re-orthogonalise when the norm dropped by more than 1/sqrt(2)
@@ -400,8 +601,8 @@ static int gmres_solve(Ctx& c, const double* A, const double* b, double* x, int 
         if (pass == 0 && hcol[k + 1] > 0.7071067811865476 * before) break;
       }
       const double hn = hcol[k + 1];
-      if (hn > 0.0)
-        hipLaunchKernelGGL(k_scale_to, g1, dim3(256), 0, c.stream, n, 1.0 / hn, vk1, vk1);
+      if (!std::isfinite(hn)) throw Error{MHX_ERR_NONFINITE, "GMRES: non-finite Krylov vector"};
+      if (hn > 0.0) hipLaunchKernelGGL(k_scale_to<T>, grid1(n), dim3(256), 0, c.stream, n, 1.0 / hn, vk1, vk1);
       for (int j = 0; j <= k + 1; ++j) H[j * m + k] = hcol[j];
       for (int j = 0; j < k; ++j) {
         const double a0 = H[j * m + k], b0 = H[(j + 1) * m + k];
@@ -425,25 +626,95 @@ static int gmres_solve(Ctx& c, const double* A, const double* b, double* x, int 
       for (int l = j + 1; l < k; ++l) s -= H[j * m + l] * y[l];
       y[j] = s / H[j * m + j];
     }
-    MHX_HIP(hipMemcpyAsync(w.scal.p, y.data(), k * sizeof(double), hipMemcpyHostToDevice, c.stream));
-    hipLaunchKernelGGL(k_gm_update, g1, dim3(256), 0, c.stream, n, k, w.V.p, n, w.scal.p, w.minv.p, x);
+    // x += M^-1 V y
+    MHX_HIP(hipMemcpyAsync(dscal + kScalY, y.data(), k * sizeof(double), hipMemcpyHostToDevice, c.stream));
+    hipLaunchKernelGGL(k_vcomb<T>, grid1(n), dim3(256), 0, c.stream, n, k, v.V.p, n, dscal + kScalY, v.tmp.p);
+    prec_apply<T>(c, P, v.tmp.p, v.tmp2.p);
+    hipLaunchKernelGGL(k_add<T>, grid1(n), dim3(256), 0, c.stream, n, v.tmp2.p, x);
     MHX_HIP(hipGetLastError());
+    // the host vector y must stay untouched until the copy has been consumed
+    MHX_HIP(hipStreamSynchronize(c.stream));
   }
   *relres = rel;
   return it;
 }
 
+// ---- FP32 inner solve inside FP64 iterative refinement -----------------------
+// r = b - A x (FP64 A, x); d = inner solve of A32 d = r / ||r|| (FP32, to a
+// relative inner_tol); x += ||r|| d; until ||r|| / ||b|| <= tol.
+static int mixed_solve(Ctx& c, const double* A, const double* b, double* x, const mhx_krylov_opts& o, int maxit,
+                       double tol, double* relres, int* refinements) {
+  const long n = c.mesh.ndof;
+  KrylovWork& w = c.kw;
+  w.jac32.alloc(c.nnz);
+  hipLaunchKernelGGL((k_cast_scale<double, float>), grid1(c.nnz), dim3(256), 0, c.stream, (long)c.nnz, 1.0, A,
+                     w.jac32.p);
+  MHX_HIP(hipGetLastError());
+  const Prec<float> P = build_prec<float>(c, A, o.precond & 0xff, w.f);
+  w.d.r.alloc(n); w.d.q.alloc(n); w.f.rhs.alloc(n); w.f.sol.alloc(n);
+  const double bn = norm2_t<double>(c, b, n);
+  if (bn == 0.0) {
+    MHX_HIP(hipMemsetAsync(x, 0, n * sizeof(double), c.stream));
+    *relres = 0.0;
+    return 0;
+  }
+  const double inner_tol = o.inner_tol > 0 ? o.inner_tol : 1e-4;
+  const int inner_max = o.inner_iters > 0 ? o.inner_iters : maxit;
+  const int m = o.restart > 0 ? o.restart : 50;
+  int total = 0, refs = 0, stalls = 0;
+  double rel = 1.0, prev = INFINITY;
+  for (;;) {
+    spmv_t<double, double>(c, A, nullptr, x, w.d.q.p);
+    hipLaunchKernelGGL(k_resid<double>, grid1(n), dim3(256), 0, c.stream, n, b, w.d.q.p, w.d.r.p);
+    const double rn = norm2_t<double>(c, w.d.r.p, n);
+    rel = rn / bn;
+    if (!std::isfinite(rel)) throw Error{MHX_ERR_NONFINITE, "refinement: non-finite residual"};
+    if (rel <= tol || total >= maxit || refs >= kMaxRefine) break;
+    stalls = rel > 0.5 * prev ? stalls + 1 : 0;
+    if (stalls >= 3) break;  // the FP32 inner solve no longer reduces the FP64 residual
+    prev = rel;
+    ScopedPhase ph(c.prof, "Krylov::refinement_sweep", c.stream);
+    hipLaunchKernelGGL((k_cast_scale<double, float>), grid1(n), dim3(256), 0, c.stream, n, 1.0 / rn, w.d.r.p,
+                       w.f.rhs.p);
+    MHX_HIP(hipMemsetAsync(w.f.sol.p, 0, n * sizeof(float), c.stream));
+    const int budget = std::max(1, std::min(inner_max, maxit - total));
+    double irel = 0.0;
+    const int it = o.solver == MHX_KRYLOV_CG
+                       ? cg_core<float>(c, w.jac32.p, P, w.f.rhs.p, w.f.sol.p, inner_tol, budget, &irel)
+                       : gmres_core<float>(c, w.jac32.p, P, w.f.rhs.p, w.f.sol.p, m, inner_tol, budget, &irel);
+    hipLaunchKernelGGL(k_axpy_f2d, grid1(n), dim3(256), 0, c.stream, n, rn, w.f.sol.p, x);
+    MHX_HIP(hipGetLastError());
+    total += it;
+    ++refs;
+    if (it == 0) break;
+  }
+  *relres = rel;
+  *refinements = refs;
+  return total;
+}
+
 int krylov_solve(Ctx& c, const double* A, const double* b, double* x, const mhx_krylov_opts& o, mhx_solve_stats* st) {
   ScopedPhase ph(c.prof, "LinearAlgebraInterface::linearSolver", c.stream);
+  const int pk = o.precond & 0xff;
+  MHX_REQUIRE(pk >= MHX_PREC_NONE && pk <= MHX_PREC_BLOCK_JACOBI && (o.precond & ~(0xff | MHX_PREC_FP32)) == 0,
+              MHX_ERR_ARG, "unknown preconditioner");
+  MHX_REQUIRE(o.solver == 0 || o.solver == MHX_KRYLOV_CG || o.solver == MHX_KRYLOV_GMRES, MHX_ERR_ARG,
+              "unknown Krylov solver");
   double rel = 0.0;
-  int it;
+  int it, refs = 0;
   const int maxit = o.max_iters > 0 ? o.max_iters : 100;
   const double tol = o.tol > 0 ? o.tol : 1e-7;
-  if (o.solver == MHX_KRYLOV_CG) it = cg_solve(c, A, b, x, tol, maxit, &rel);
-  else it = gmres_solve(c, A, b, x, o.restart > 0 ? o.restart : 50, tol, maxit, &rel);
+  if (o.precond & MHX_PREC_FP32) {
+    it = mixed_solve(c, A, b, x, o, maxit, tol, &rel, &refs);
+  } else {
+    const Prec<double> P = build_prec<double>(c, A, pk, c.kw.d);
+    if (o.solver == MHX_KRYLOV_CG) it = cg_core<double>(c, A, P, b, x, tol, maxit, &rel);
+    else it = gmres_core<double>(c, A, P, b, x, o.restart > 0 ? o.restart : 50, tol, maxit, &rel);
+  }
   if (st) {
     st->iters += it;
     st->rel_res = rel;
+    st->refinements += refs;
   }
   if (!(rel <= tol)) return MHX_ERR_NOT_CONVERGED;
   return MHX_OK;

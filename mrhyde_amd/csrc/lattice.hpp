@@ -1,0 +1,74 @@
+// lattice.hpp — element / node arithmetic on the structured box lattice and
+// the colour schedule shared by the assembly kernels.
+#pragma once
+
+#include "mhx_internal.hpp"
+
+namespace mhx {
+
+// lattice helpers ------------------------------------------------------------
+template <int DIM, int P>
+__device__ __forceinline__ long node_of(const BoxMesh& m, const int* ec, int j) {
+  constexpr int P1 = P + 1;
+  int jx = j % P1, jy = (j / P1) % P1, jz = DIM == 3 ? j / (P1 * P1) : 0;
+  return (long)(P * ec[0] + jx) + (long)m.nn[0] * ((long)(P * ec[1] + jy) + (long)m.nn[1] * (long)(P * ec[2] + jz));
+}
+
+// position of column-node offset (local jl) inside the row of local node il,
+// along one axis: returns pos and the 1-D row length.
+template <int P>
+__device__ __forceinline__ void axis_slot(int il, int jl, int ecd, int nnd, int& pos, int& len) {
+  if (il == 0 || il == P) {
+    int r = P * ecd + il;
+    int lo = r - P < 0 ? 0 : r - P;
+    int hi = r + P > nnd - 1 ? nnd - 1 : r + P;
+    pos = P * ecd + jl - lo;
+    len = hi - lo + 1;
+  } else {
+    pos = jl;
+    len = P + 1;
+  }
+}
+
+// Element owned by group g of this workgroup in sub-phase `sub`.
+//   element colouring (a.blocked == 0): one launch per colour, one element per
+//     group, elements of the colour enumerated over the grid;
+//   block colouring (a.blocked == 1): the mesh is tiled by blocks of
+//     bw = 2*s elements per axis; one launch per block colour, one block per
+//     workgroup; sub-phase `sub` (an element colour) gives group g the element
+//     block_origin + 2*g_d + sub_d.  Same-colour elements of one block share no
+//     DOF, and same-colour blocks are a block apart, so no two concurrent
+//     element scatters touch the same CRS entry.
+template <int DIM>
+__device__ __forceinline__ bool element_of(const AsmArgs& a, int g, int sub, int* ec) {
+  ec[0] = ec[1] = ec[2] = 0;
+  if (!a.blocked) {
+    const int idx = (int)blockIdx.x * a.gpb + g;
+    int t = idx < a.ncolor ? idx : 0;
+    ec[0] = 2 * (t % a.cn[0]) + (a.color & 1);
+    t /= a.cn[0];
+    ec[1] = 2 * (t % a.cn[1]) + ((a.color >> 1) & 1);
+    t /= a.cn[1];
+    if (DIM == 3) ec[2] = 2 * t + ((a.color >> 2) & 1);
+    return idx < a.ncolor;
+  }
+  int t = blockIdx.x;
+  const int gc[3] = {g % a.s[0], (g / a.s[0]) % a.s[1], g / (a.s[0] * a.s[1])};
+  bool ok = true;
+  for (int d = 0; d < DIM; ++d) {
+    const int bi = 2 * (t % a.cn[d]) + ((a.color >> d) & 1);
+    t /= a.cn[d];
+    ec[d] = bi * a.bw[d] + 2 * gc[d] + ((sub >> d) & 1);
+    ok = ok && ec[d] < a.m.nel[d];
+  }
+  return ok && g < a.gpb;
+}
+
+// Scheduling key of element index e along one axis: (block colour bit,
+// element colour bit).  Launch order is block colour major, sub-phase minor,
+// so along each axis the smaller key runs first.
+__device__ __forceinline__ int axis_key(const AsmArgs& a, int d, int e) {
+  return (a.blocked ? ((e / a.bw[d]) & 1) * 2 : 0) + (e & 1);
+}
+
+}  // namespace mhx

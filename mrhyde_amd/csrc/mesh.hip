@@ -1,5 +1,6 @@
-// mesh_host.cpp — structured box mesh, DOF numbering, reference 1-D rules,
-// strong-Dirichlet tables and the CRS row pointer (host side of setup).
+// mesh.hip — structured box mesh, DOF numbering, reference 1-D rules,
+// strong-Dirichlet tables and the CRS graph (row pointer on the host, column
+// indices by a device kernel).
 //
 // Replaces the setup half of the reference path:
 //   MeshInterface inline Square/Cube factories (src/interfaces/mesh/meshInterface_construct.hpp:134-200)
@@ -85,10 +86,13 @@ static inline double u01(uint64_t seed, uint64_t idx) {
   return (double)(z >> 11) * (1.0 / 9007199254740992.0);
 }
 
+// One thread per DOF row: the row's columns are the nf unknowns of every
+// lattice node in the tensor range [lo_d, hi_d], node-major, field-minor.
 __global__ void k_fill_colidx(BoxMesh m, const int32_t* __restrict__ rowptr, int32_t* __restrict__ colidx) {
   long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= m.ndof) return;
-  int c[3] = {(int)(r % m.nn[0]), (int)((r / m.nn[0]) % m.nn[1]), (int)(r / ((long)m.nn[0] * m.nn[1]))};
+  const long node = r / m.nf;
+  int c[3] = {(int)(node % m.nn[0]), (int)((node / m.nn[0]) % m.nn[1]), (int)(node / ((long)m.nn[0] * m.nn[1]))};
   int lo[3], hi[3];
   for (int a = 0; a < 3; ++a) {
     if (a >= m.dim) { lo[a] = hi[a] = 0; continue; }
@@ -103,7 +107,10 @@ __global__ void k_fill_colidx(BoxMesh m, const int32_t* __restrict__ rowptr, int
   long k = rowptr[r];
   for (int z = lo[2]; z <= hi[2]; ++z)
     for (int y = lo[1]; y <= hi[1]; ++y)
-      for (int x = lo[0]; x <= hi[0]; ++x) colidx[k++] = (int32_t)(x + (long)m.nn[0] * (y + (long)m.nn[1] * z));
+      for (int x = lo[0]; x <= hi[0]; ++x) {
+        const long nb = x + (long)m.nn[0] * (y + (long)m.nn[1] * z);
+        for (int f = 0; f < m.nf; ++f) colidx[k++] = (int32_t)(nb * m.nf + f);
+      }
 }
 
 static inline long row_len_1d(int c, int p, int n) {
@@ -125,15 +132,19 @@ void build_box(Ctx& c) {
   m.nel[0] = d.nx;
   m.nel[1] = d.ny;
   m.nel[2] = d.dim == 3 ? d.nz : 1;
-  m.ne = m.ndof = m.nvert = 1;
+  const bool ns = d.physics == MHX_PHYS_NAVIER_STOKES;
+  m.nf = ns ? m.dim + 1 : 1;
+  c.nfield = m.nf;
+  m.ne = m.nnode = m.nvert = 1;
   for (int a = 0; a < 3; ++a) {
     bool act = a < m.dim;
     m.nn[a] = act ? m.p * m.nel[a] + 1 : 1;
     m.nvx[a] = act ? m.nel[a] + 1 : 1;
     m.ne *= m.nel[a];
-    m.ndof *= m.nn[a];
+    m.nnode *= m.nn[a];
     m.nvert *= m.nvx[a];
   }
+  m.ndof = m.nnode * m.nf;
   MHX_REQUIRE(m.ndof < (1L << 31) - 1, MHX_ERR_UNSUPPORTED, "more than 2^31 DOFs");
   c.nb = 1;
   c.nq = 1;
@@ -174,29 +185,48 @@ void build_box(Ctx& c) {
       c.h_vx[v * 3 + a] = x;
     }
   }
-  // Dirichlet mask and values
+  // Dirichlet mask and values.  Thermal: the DOFs on dbc_sides.  Navier-
+  // Stokes: velocity component c on dbc_sides | ns_lid_sides |
+  // ns_comp_sides[c] (ns_lid_u on a lid side, edges included, else 0) and
+  // optionally the pressure at node 0.
   c.h_fixed.assign(m.ndof, 0);
   c.h_dbc.assign(m.ndof, 0.0);
   c.h_dbc_list.clear();
-  for (long g = 0; g < m.ndof; ++g) {
-    long gc[3] = {g % m.nn[0], (g / m.nn[0]) % m.nn[1], g / ((long)m.nn[0] * m.nn[1])};
-    bool f = false;
+  const uint32_t vel_sides = d.dbc_sides | (ns ? d.ns_lid_sides : 0u);
+  for (long node = 0; node < m.nnode; ++node) {
+    long gc[3] = {node % m.nn[0], (node / m.nn[0]) % m.nn[1], node / ((long)m.nn[0] * m.nn[1])};
+    uint32_t on = 0;
     for (int a = 0; a < m.dim; ++a) {
-      if ((d.dbc_sides & (1u << (2 * a))) && gc[a] == 0) f = true;
-      if ((d.dbc_sides & (1u << (2 * a + 1))) && gc[a] == m.nn[a] - 1) f = true;
+      if (gc[a] == 0) on |= 1u << (2 * a);
+      if (gc[a] == m.nn[a] - 1) on |= 1u << (2 * a + 1);
     }
-    if (f) {
-      c.h_fixed[g] = 1;
-      c.h_dbc[g] = d.dbc_values ? d.dbc_values[g] : d.dbc_const;
-      c.h_dbc_list.push_back((int32_t)g);
+    for (int f = 0; f < m.nf; ++f) {
+      const long g = node * m.nf + f;
+      bool fix = false;
+      double val = 0.0;
+      if (!ns) {
+        fix = (on & d.dbc_sides) != 0;
+        val = d.dbc_values ? d.dbc_values[g] : d.dbc_const;
+      } else if (f < m.dim) {
+        fix = (on & (vel_sides | d.ns_comp_sides[f])) != 0;
+        val = (on & d.ns_lid_sides) ? d.ns_lid_u[f] : 0.0;
+      } else {
+        fix = d.ns_pin_pressure && node == 0;
+      }
+      if (fix) {
+        c.h_fixed[g] = 1;
+        c.h_dbc[g] = val;
+        c.h_dbc_list.push_back((int32_t)g);
+      }
     }
   }
   // CRS row pointer
   c.h_rowptr.assign(m.ndof + 1, 0);
   long acc = 0;
   for (long g = 0; g < m.ndof; ++g) {
-    long gc[3] = {g % m.nn[0], (g / m.nn[0]) % m.nn[1], g / ((long)m.nn[0] * m.nn[1])};
-    long len = 1;
+    const long node = g / m.nf;
+    long gc[3] = {node % m.nn[0], (node / m.nn[0]) % m.nn[1], node / ((long)m.nn[0] * m.nn[1])};
+    long len = m.nf;
     for (int a = 0; a < m.dim; ++a) len *= row_len_1d((int)gc[a], m.p, m.nn[a]);
     c.h_rowptr[g] = (int32_t)acc;
     acc += len;

@@ -51,10 +51,11 @@ struct Tables1D {
 // p*e + {0..p} per axis; rows are lexicographic in the node lattice).
 struct BoxMesh {
   int dim, p, nq1;
+  int nf;         // unknowns per node; DOF = node * nf + field
   int nel[3];     // elements per axis
   int nn[3];      // DOF nodes per axis (p*nel+1)
   int nvx[3];     // vertices per axis (nel+1)
-  long ne, ndof, nvert;
+  long ne, ndof, nvert, nnode;
 };
 
 // Thermal coefficients (compiled form of "thermal diffusion" k0_e*(1+a*T^2)
@@ -160,12 +161,17 @@ struct ScopedPhase {
 // ---------------------------------------------------------------------------
 // Krylov workspace
 // ---------------------------------------------------------------------------
+template <class T>
+struct KrylovVecs {
+  DevBuf<T> r, z, p, q, tmp, tmp2, V, prec;  // prec: 1/diag or nodal block inverses
+  DevBuf<T> rhs, sol;                         // inner system of the refinement loop
+};
 struct KrylovWork {
-  DevBuf<double> r, z, p, q, x, w, tmp, minv, scal, V;
-  DevBuf<float> jac32, minv32, r32, x32, V32, w32, tmp32;
-  DevBuf<double> part;
-  DevBuf<unsigned int> counter;
-  int m = 0;
+  KrylovVecs<double> d;
+  KrylovVecs<float> f;
+  DevBuf<float> jac32;
+  DevBuf<double> x64;  // FP64 residual / correction of the refinement loop
+  DevBuf<double> part, scal;
 };
 
 // ---------------------------------------------------------------------------
@@ -196,11 +202,12 @@ struct Ctx {
   bool own_stream = false;
   Profiler prof;
   int spmv_lanes = 16;
+  int nfield = 1;  // unknowns per node (consecutive DOFs): block size of block Jacobi
   int scatter_mode = kElementColouring;
   std::string err;
 };
 
-// mesh / graph (mesh_host.cpp)
+// mesh / graph (mesh.hip)
 void build_box(Ctx& c);
 void gauss_legendre_desc(int n, double* x, double* w);
 void lagrange_equispaced(int p, double xi, double* L, double* dL);
@@ -209,6 +216,9 @@ void lagrange_equispaced(int p, double xi, double* L, double* dL);
 bool thermal_supported(int dim, int p, int nq1);
 void assemble_thermal(Ctx& c, const double* u, double* res, double* jac, bool want_res, bool want_jac,
                       bool accumulate);
+// navier stokes assembly (ns.hip)
+bool ns_supported(int dim, int p, int nq1);
+void assemble_ns(Ctx& c, const double* u, double* res, double* jac, bool want_res, bool want_jac, bool accumulate);
 
 // linear algebra (krylov.hip)
 void spmv(Ctx& c, const double* A, const double* x, double* y);

@@ -150,6 +150,19 @@ class Mhx:
         self._check(lib.mhx_assemble(self._h, C.c_void_p(pu), flags, C.c_void_p(pr) if pr else None,
                                      C.c_void_p(pj) if pj else None), "assemble_resident")
 
+    def solve_resident(self, opts: Optional[capi.KrylovOpts] = None, **kw):
+        """Solve J du = res on the context's device buffers (J, res from
+        assemble_resident; du is used as the initial guess)."""
+        opts = opts or krylov_opts(**kw)
+        opts.flags |= capi.MHX_DEVICE_PTRS
+        pj, pr, px = (self.state_ptr(b)[0] for b in (capi.MHX_BUF_JAC, capi.MHX_BUF_RES, capi.MHX_BUF_DU))
+        st = capi.SolveStats()
+        rc = lib.mhx_linear_solve(self._h, C.c_void_p(pj), C.c_void_p(pr), C.c_void_p(px), C.byref(opts),
+                                  C.byref(st))
+        if rc < 0 and rc != capi.MHX_ERR_NOT_CONVERGED:
+            self._check(rc, "solve_resident")
+        return st.as_dict()
+
     # ---- setup products -------------------------------------------------------
     def crs_pattern(self):
         rp = np.empty(self.n_dof + 1, dtype=np.int32)

@@ -100,7 +100,7 @@ class Oracle:
         return u
 
     def lids(self):
-        L = np.empty((self.n_elem, self.nb), dtype=np.int32)
+        L = np.empty((self.n_elem, self.nb * self.problem.n_field), dtype=np.int32)
         lib.orc_lids(self._h, L.ctypes.data)
         return L
 

@@ -59,7 +59,11 @@ template <int N> inline SFad<N> operator-(double a, const SFad<N>& b) { SFad<N> 
 template <int N> inline SFad<N> operator*(const SFad<N>& a, double b) { SFad<N> r; r.v = a.v * b; for (int i = 0; i < N; ++i) r.d[i] = a.d[i] * b; return r; }
 template <int N> inline SFad<N> operator*(double a, const SFad<N>& b) { SFad<N> r; r.v = a * b.v; for (int i = 0; i < N; ++i) r.d[i] = a * b.d[i]; return r; }
 template <int N> inline SFad<N> operator-(const SFad<N>& a) { SFad<N> r; r.v = -a.v; for (int i = 0; i < N; ++i) r.d[i] = -a.d[i]; return r; }
+template <int N> inline SFad<N> operator/(const SFad<N>& a, double b) { SFad<N> r; r.v = a.v / b; for (int i = 0; i < N; ++i) r.d[i] = a.d[i] / b; return r; }
+template <int N> inline SFad<N> operator/(double a, const SFad<N>& b) { SFad<N> r; r.v = a / b.v; for (int i = 0; i < N; ++i) r.d[i] = -a * b.d[i] / (b.v * b.v); return r; }
 template <int N> inline SFad<N> sqrt(const SFad<N>& a) { SFad<N> r; r.v = std::sqrt(a.v); double s = 0.5 / r.v; for (int i = 0; i < N; ++i) r.d[i] = a.d[i] * s; return r; }
+inline double dsqrt(double a) { return std::sqrt(a); }
+template <int N> inline SFad<N> dsqrt(const SFad<N>& a) { return sqrt(a); }
 
 template <int N> inline double val(const SFad<N>& a) { return a.v; }
 template <int N> inline double dx(const SFad<N>& a, int i) { return a.d[i]; }
@@ -148,12 +152,13 @@ static inline double rng_u01(uint64_t seed, uint64_t idx) {
 // ---------------------------------------------------------------------------
 struct Problem {
   mhx_problem_desc desc;
-  int dim, p, nq1, nb, nq, nv;       // nb = DOFs/elem, nq = points/elem, nv = vertices/elem
+  int dim, p, nq1, nb, nq, nv;       // nb = basis functions/elem, nq = points/elem, nv = vertices/elem
+  int nf;                            // unknowns per node (thermal 1; Navier-Stokes dim+1)
   int nel[3], nn[3], nvx[3];         // elements, DOF nodes, vertices per axis
-  long ne, ndof, nvert;
+  long ne, ndof, nvert, nnode;       // ndof = nnode * nf, DOF = node * nf + field
   std::vector<double> k0, src;       // per element
   std::vector<double> vx;            // vertex coords [nvert*3]
-  std::vector<int> lids;             // [ne*nb]
+  std::vector<int> lids;             // [ne*nf*nb], (e, field, basis)
   std::vector<double> qx1, qw1;      // 1-D Gauss points/weights
   std::vector<double> bref, gref;    // [nb*nq], [nb*nq*dim] reference basis / grads
   std::vector<double> qwref;         // [nq]
@@ -185,17 +190,19 @@ static Problem* build_problem(const mhx_problem_desc* d) {
   P->nb = 1; P->nq = 1; P->nv = 1;
   for (int a = 0; a < dim; ++a) { P->nb *= (p + 1); P->nq *= P->nq1; P->nv *= 2; }
   P->nel[0] = d->nx; P->nel[1] = d->ny; P->nel[2] = dim == 3 ? d->nz : 1;
-  P->ne = 1; P->ndof = 1; P->nvert = 1;
+  P->nf = d->physics == MHX_PHYS_NAVIER_STOKES ? dim + 1 : 1;
+  P->ne = 1; P->nnode = 1; P->nvert = 1;
   for (int a = 0; a < 3; ++a) {
     bool act = a < dim;
     P->nn[a] = act ? p * P->nel[a] + 1 : 1;
     P->nvx[a] = act ? P->nel[a] + 1 : 1;
     P->ne *= P->nel[a];
-    P->ndof *= P->nn[a];
+    P->nnode *= P->nn[a];
     P->nvert *= P->nvx[a];
   }
+  P->ndof = P->nnode * P->nf;
   // reference AD size selection (assemblyManager_construct.hpp:128-171)
-  int nd = P->nb;
+  int nd = P->nb * P->nf;
   P->ad_size = nd <= 2 ? 2 : nd <= 4 ? 4 : nd <= 8 ? 8 : nd <= 16 ? 16 : nd <= 18 ? 18 : nd <= 24 ? 24 : nd <= 32 ? 32 : 64;
 
   P->k0.assign(P->ne, 1.0);
@@ -218,8 +225,10 @@ static Problem* build_problem(const mhx_problem_desc* d) {
       P->vx[v * 3 + a] = x;
     }
   }
-  // connectivity, lexicographic local order
-  P->lids.assign(P->ne * P->nb, 0);
+  // connectivity, lexicographic local order; per-field element offsets
+  // f*nb + j (the role of wkset->offsets(var, dof))
+  const int nf = P->nf;
+  P->lids.assign(P->ne * nf * P->nb, 0);
   for (long e = 0; e < P->ne; ++e) {
     long c[3];
     P->elem_coords(e, c);
@@ -227,7 +236,8 @@ static Problem* build_problem(const mhx_problem_desc* d) {
       int jl[3] = {j % (p + 1), (j / (p + 1)) % (p + 1), dim == 3 ? j / ((p + 1) * (p + 1)) : 0};
       long g[3];
       for (int a = 0; a < 3; ++a) g[a] = a < dim ? p * c[a] + jl[a] : 0;
-      P->lids[e * P->nb + j] = (int)(g[0] + P->nn[0] * (g[1] + (long)P->nn[1] * g[2]));
+      const long node = g[0] + P->nn[0] * (g[1] + (long)P->nn[1] * g[2]);
+      for (int f = 0; f < nf; ++f) P->lids[(e * nf + f) * P->nb + j] = (int)(node * nf + f);
     }
   }
   // 1-D rules and tensor tables
@@ -274,10 +284,11 @@ static Problem* build_problem(const mhx_problem_desc* d) {
   // CRS graph: union over elements of (gid_i x all gids of element), sorted
   // (linearAlgebraInterface_construct.hpp:243-265).
   std::vector<std::vector<int>> rows(P->ndof);
+  const int nbd = nb * nf;
   for (long e = 0; e < P->ne; ++e) {
-    const int* L = &P->lids[e * nb];
-    for (int i = 0; i < nb; ++i)
-      for (int j = 0; j < nb; ++j) rows[L[i]].push_back(L[j]);
+    const int* L = &P->lids[e * nbd];
+    for (int i = 0; i < nbd; ++i)
+      for (int j = 0; j < nbd; ++j) rows[L[i]].push_back(L[j]);
   }
   P->rowptr.assign(P->ndof + 1, 0);
   for (long r = 0; r < P->ndof; ++r) {
@@ -291,18 +302,35 @@ static Problem* build_problem(const mhx_problem_desc* d) {
     std::copy(rows[r].begin(), rows[r].end(), P->colidx.begin() + P->rowptr[r]);
   // strong Dirichlet DOFs on the requested sides (setDirichletData,
   // discretizationInterface_dof.hpp:643-736) and their values.
+  // Navier-Stokes: velocity component c on dbc_sides | ns_lid_sides |
+  // ns_comp_sides[c] (lid value on the lid sides, 0 elsewhere), pressure
+  // optionally pinned at node 0.
   P->fixed.assign(P->ndof, 0);
   P->dbcval.assign(P->ndof, 0.0);
-  for (long g = 0; g < P->ndof; ++g) {
-    long c[3] = {g % P->nn[0], (g / P->nn[0]) % P->nn[1], g / ((long)P->nn[0] * P->nn[1])};
-    bool f = false;
+  const bool ns = d->physics == MHX_PHYS_NAVIER_STOKES;
+  const uint32_t vel_sides = d->dbc_sides | (ns ? d->ns_lid_sides : 0u);
+  for (long node = 0; node < P->nnode; ++node) {
+    long c[3] = {node % P->nn[0], (node / P->nn[0]) % P->nn[1], node / ((long)P->nn[0] * P->nn[1])};
+    uint32_t on = 0;  // sides this node lies on
     for (int a = 0; a < dim; ++a) {
-      if ((d->dbc_sides & (1u << (2 * a))) && c[a] == 0) f = true;
-      if ((d->dbc_sides & (1u << (2 * a + 1))) && c[a] == P->nn[a] - 1) f = true;
+      if (c[a] == 0) on |= 1u << (2 * a);
+      if (c[a] == P->nn[a] - 1) on |= 1u << (2 * a + 1);
     }
-    if (f) {
-      P->fixed[g] = 1;
-      P->dbcval[g] = d->dbc_values ? d->dbc_values[g] : d->dbc_const;
+    for (int f = 0; f < nf; ++f) {
+      const long g = node * nf + f;
+      if (!ns) {
+        if (on & d->dbc_sides) {
+          P->fixed[g] = 1;
+          P->dbcval[g] = d->dbc_values ? d->dbc_values[g] : d->dbc_const;
+        }
+      } else if (f < dim) {
+        if (on & (vel_sides | d->ns_comp_sides[f])) {
+          P->fixed[g] = 1;
+          P->dbcval[g] = (on & d->ns_lid_sides) ? d->ns_lid_u[f] : 0.0;
+        }
+      } else if (d->ns_pin_pressure && node == 0) {
+        P->fixed[g] = 1;
+      }
     }
   }
   return P;
@@ -461,17 +489,150 @@ static void assemble_thermal(const Problem& P, const double* u, bool want_jac, d
       if (P.fixed[g]) jv[find_slot(P, (int)g, (int)g)] = 1.0;
 }
 
+// navierstokes<EvalT>::computeTau (navierstokes.cpp:1053-1081), steady
+// (C3 = 0): tau = 1 / sqrt((C1 nu / h^2)^2 + (C2 |u| / h)^2), where |u|^2 is
+// replaced by its square root only when it exceeds 1e-12, as there.
+template <class EvalT>
+static EvalT ns_tau(double visc, const EvalT* vel, int dim, double h) {
+  const double C1 = 4.0, C2 = 2.0;
+  EvalT nvel = vel[0] * vel[0];
+  for (int a = 1; a < dim; ++a) nvel = nvel + vel[a] * vel[a];
+  if (val(nvel) > 1e-12) nvel = dsqrt(nvel);
+  const double t1 = (C1 * visc / h / h) * (C1 * visc / h / h);
+  EvalT t2 = (C2 * nvel / h) * (C2 * nvel / h);
+  EvalT tau = t2 + t1;
+  return 1.0 / dsqrt(tau);
+}
+
+// assembleJacRes / assembleRes for the navier stokes module: the workset loop
+// as for thermal, the fields ux, uy, [uz], pr and their gradients
+// (workset.cpp:979-1115), and navierstokes<EvalT>::volumeResidual for the
+// steady case without energy coupling (navierstokes.cpp:252-494 in 2-D,
+// :494-~850 in 3-D): Galerkin momentum and continuity, SUPG (useSUPG) and
+// PSPG (usePSPG) with the element size of Workset::getElementSize
+// (workset.cpp:2699-2712), h = (sum_q wts)^(1/dim).
+template <class EvalT>
+static void assemble_ns(const Problem& P, const double* u, bool want_jac, double* res, double* jv) {
+  const int ws = P.desc.workset_size > 0 ? P.desc.workset_size : 100;
+  const int nb = P.nb, nq = P.nq, dim = P.dim, nf = P.nf, nbd = nb * nf;
+  const double rho = P.desc.ns_rho, visc = P.desc.ns_mu;
+  const double* force = P.desc.ns_force;
+  std::vector<EvalT> uAD, U, dU;
+  ElemGeom G;
+  for (long e0 = 0; e0 < P.ne; e0 += ws) {
+    const long nel = std::min<long>(ws, P.ne - e0);
+    uAD.assign(nel * nbd, EvalT(0.0));
+    for (long le = 0; le < nel; ++le)
+      for (int k = 0; k < nbd; ++k) {
+        const double uv = u[P.lids[(e0 + le) * nbd + k]];
+        uAD[le * nbd + k] = want_jac ? Seed<EvalT>::make(k, uv) : EvalT(uv);
+      }
+    for (long le = 0; le < nel; ++le) {
+      const long e = e0 + le;
+      elem_geometry(P, e, G);
+      // fields U[f][q] and gradients dU[(f*nq + q)*dim + a]
+      U.assign(nf * nq, EvalT(0.0));
+      dU.assign(nf * nq * dim, EvalT(0.0));
+      for (int f = 0; f < nf; ++f)
+        for (int q = 0; q < nq; ++q) {
+          const EvalT* uf = &uAD[le * nbd + f * nb];
+          U[f * nq + q] = uf[0] * P.bref[q];
+          for (int a = 0; a < dim; ++a) dU[(f * nq + q) * dim + a] = uf[0] * G.grad[(0 * nq + q) * dim + a];
+          for (int j = 1; j < nb; ++j) {
+            U[f * nq + q] += uf[j] * P.bref[j * nq + q];
+            for (int a = 0; a < dim; ++a) dU[(f * nq + q) * dim + a] += uf[j] * G.grad[(j * nq + q) * dim + a];
+          }
+        }
+      double vol = 0.0;
+      for (int q = 0; q < nq; ++q) vol += G.wts[q];
+      const double h = std::pow(vol, 1.0 / (double)dim);
+      const int pf = dim;  // pressure field
+      std::vector<EvalT> r(nbd, EvalT(0.0));
+      auto gradU = [&](int f, int q, int a) -> const EvalT& { return dU[(f * nq + q) * dim + a]; };
+      for (int q = 0; q < nq; ++q) {
+        const double w = G.wts[q];
+        EvalT vel[3] = {EvalT(0.0), EvalT(0.0), EvalT(0.0)};
+        for (int a = 0; a < dim; ++a) vel[a] = U[a * nq + q];
+        const EvalT& pr = U[pf * nq + q];
+        const bool stab = P.desc.ns_supg || P.desc.ns_pspg;
+        const EvalT tau = stab ? ns_tau<EvalT>(visc, vel, dim, h) : EvalT(0.0);
+        // momentum residual R_a = rho (u . grad u_a) + d_a p - rho f_a (steady)
+        EvalT conv[3], stabres[3];
+        for (int a = 0; a < dim; ++a) {
+          conv[a] = vel[0] * gradU(a, q, 0);
+          for (int c = 1; c < dim; ++c) conv[a] = conv[a] + vel[c] * gradU(a, q, c);
+          stabres[a] = rho * conv[a] + gradU(pf, q, a) - rho * force[a];
+        }
+        for (int a = 0; a < dim; ++a) {
+          EvalT Fl[3];
+          for (int c = 0; c < dim; ++c) {
+            Fl[c] = visc * gradU(a, q, c);
+            if (c == a) Fl[c] = Fl[c] - pr;
+            Fl[c] = Fl[c] * w;
+          }
+          EvalT F = (conv[a] - force[a]) * (rho * w);
+          for (int i = 0; i < nb; ++i) {
+            EvalT t = F * P.bref[i * nq + q];
+            for (int c = 0; c < dim; ++c) t += Fl[c] * G.grad[(i * nq + q) * dim + c];
+            r[a * nb + i] += t;
+          }
+          if (P.desc.ns_supg) {
+            for (int i = 0; i < nb; ++i) {
+              EvalT t(0.0);
+              for (int c = 0; c < dim; ++c) t += tau * stabres[a] * vel[c] * w * G.grad[(i * nq + q) * dim + c];
+              r[a * nb + i] += t;
+            }
+          }
+        }
+        // continuity: div u, and PSPG tau/rho R . grad q
+        EvalT divu = gradU(0, q, 0);
+        for (int c = 1; c < dim; ++c) divu = divu + gradU(c, q, c);
+        divu = divu * w;
+        for (int i = 0; i < nb; ++i) r[pf * nb + i] += divu * P.bref[i * nq + q];
+        if (P.desc.ns_pspg) {
+          EvalT S[3];
+          for (int c = 0; c < dim; ++c) S[c] = stabres[c] * (tau * w / rho);
+          for (int i = 0; i < nb; ++i) {
+            EvalT t(0.0);
+            for (int c = 0; c < dim; ++c) t += S[c] * G.grad[(i * nq + q) * dim + c];
+            r[pf * nb + i] += t;
+          }
+        }
+      }
+      // scatter (assemblyManager_scatter.hpp:164-278)
+      const int* L = &P.lids[e * nbd];
+      for (int i = 0; i < nbd; ++i) {
+        const int row = L[i];
+        if (P.fixed[row]) continue;
+        if (res) res[row] += -val(r[i]);
+        if (want_jac && jv)
+          for (int j = 0; j < nbd; ++j) jv[find_slot(P, row, L[j])] += dx(r[i], j);
+      }
+    }
+  }
+  if (want_jac && jv)
+    for (long g = 0; g < P.ndof; ++g)
+      if (P.fixed[g]) jv[find_slot(P, (int)g, (int)g)] = 1.0;
+}
+
 static void assemble(const Problem& P, const double* u, bool want_jac, double* res, double* jv) {
-  if (!want_jac) { assemble_thermal<double>(P, u, false, res, nullptr); return; }
+  const bool ns = P.desc.physics == MHX_PHYS_NAVIER_STOKES;
+  auto run = [&](auto tag) {
+    using EvalT = typename decltype(tag)::type;
+    if (ns) assemble_ns<EvalT>(P, u, want_jac, res, jv);
+    else assemble_thermal<EvalT>(P, u, want_jac, res, jv);
+  };
+  struct D { using type = double; };
+  if (!want_jac) { run(D{}); return; }
   switch (P.ad_size) {
-    case 2:  assemble_thermal<SFad<2>>(P, u, true, res, jv); break;
-    case 4:  assemble_thermal<SFad<4>>(P, u, true, res, jv); break;
-    case 8:  assemble_thermal<SFad<8>>(P, u, true, res, jv); break;
-    case 16: assemble_thermal<SFad<16>>(P, u, true, res, jv); break;
-    case 18: assemble_thermal<SFad<18>>(P, u, true, res, jv); break;
-    case 24: assemble_thermal<SFad<24>>(P, u, true, res, jv); break;
-    case 32: assemble_thermal<SFad<32>>(P, u, true, res, jv); break;
-    default: assemble_thermal<SFad<64>>(P, u, true, res, jv); break;
+    case 2:  { struct T { using type = SFad<2>; }; run(T{}); break; }
+    case 4:  { struct T { using type = SFad<4>; }; run(T{}); break; }
+    case 8:  { struct T { using type = SFad<8>; }; run(T{}); break; }
+    case 16: { struct T { using type = SFad<16>; }; run(T{}); break; }
+    case 18: { struct T { using type = SFad<18>; }; run(T{}); break; }
+    case 24: { struct T { using type = SFad<24>; }; run(T{}); break; }
+    case 32: { struct T { using type = SFad<32>; }; run(T{}); break; }
+    default: { struct T { using type = SFad<64>; }; run(T{}); break; }
   }
 }
 
@@ -693,7 +854,14 @@ int orc_create(const mhx_problem_desc* d, void** out) {
     orc::g_err = "orc_create: bad description";
     return MHX_ERR_ARG;
   }
-  if (d->physics != MHX_PHYS_THERMAL) { orc::g_err = "orc_create: only thermal"; return MHX_ERR_UNSUPPORTED; }
+  if (d->physics != MHX_PHYS_THERMAL && d->physics != MHX_PHYS_NAVIER_STOKES) {
+    orc::g_err = "orc_create: unknown physics";
+    return MHX_ERR_UNSUPPORTED;
+  }
+  if (d->physics == MHX_PHYS_NAVIER_STOKES && !(d->ns_rho > 0.0 && d->ns_mu > 0.0)) {
+    orc::g_err = "orc_create: density and viscosity must be positive";
+    return MHX_ERR_ARG;
+  }
   *out = orc::build_problem(d);
   return MHX_OK;
 }
@@ -768,7 +936,8 @@ int orc_dof_coords(void* h, double* xyz) {
   auto* P = (orc::Problem*)h;
   int p = P->p;
   for (long g = 0; g < P->ndof; ++g) {
-    long c[3] = {g % P->nn[0], (g / P->nn[0]) % P->nn[1], g / ((long)P->nn[0] * P->nn[1])};
+    const long node = g / P->nf;
+    long c[3] = {node % P->nn[0], (node / P->nn[0]) % P->nn[1], node / ((long)P->nn[0] * P->nn[1])};
     long ec[3] = {0, 0, 0};
     double xi[3] = {0, 0, 0};
     for (int a = 0; a < P->dim; ++a) {

@@ -49,3 +49,35 @@ def assert_entries_close(a, b, rowptr, rel=1e-12, floor=1e-14, what="entries"):
         i = int(np.argmax(err / np.maximum(bound, 1e-300)))
         raise AssertionError(f"{what}: {int(bad.sum())} entries out of tolerance; worst at {i}: "
                              f"{a[i]!r} vs {b[i]!r} (bound {bound[i]:.3e})")
+
+
+def field_l2_errors(oracle, u, exact):
+    """L2 error of every field at the volume cubature points, as the reference's
+    PostprocessManager::computeError does (sum over elements and points of
+    w |det J| (u_h - u_true)^2, square root at print); exact[f](x) -> values.
+    Uses the oracle's physical basis and weights (checker only)."""
+    nf = oracle.problem.n_field
+    L = oracle.lids()
+    err = np.zeros(nf)
+    for e in range(oracle.n_elem):
+        phi, _, wts, x = oracle.elem_basis(e)
+        for f in range(nf):
+            uh = u[L[e, f * oracle.nb:(f + 1) * oracle.nb]] @ phi
+            err[f] += np.sum(wts * (uh - exact[f](x)) ** 2)
+    return np.sqrt(err)
+
+
+def ns_channel_problem(deck, nz=0, depth=0.0):
+    """The channel deck (tests/golden/ns_gold.json); nz > 0 extrudes it in z
+    with uz = 0 on the z faces (ux, uy free there)."""
+    from mrhyde_amd.abi import MHX_PHYS_NAVIER_STOKES, Problem
+
+    n, lo, hi = list(deck["n"]), list(deck["lo"]), list(deck["hi"])
+    dim = deck["dim"]
+    comp = (0, 0, 0)
+    if nz:
+        dim, n, lo, hi = 3, n + [nz], lo + [0.0], hi + [depth]
+        comp = (0, 0, 16 | 32)
+    return Problem(dim=dim, order=deck["order"], quadrature=deck["quadrature"], n=n, lo=lo, hi=hi,
+                   physics=MHX_PHYS_NAVIER_STOKES, dbc_sides=deck["vel_dbc_sides"], rho=deck["rho"],
+                   mu=deck["mu"], pspg=deck["pspg"], supg=deck["supg"], force=deck["force"], comp_sides=comp)

@@ -3,7 +3,8 @@
 Reads (from a MrHyDE checkout, default /root/reference):
   regression/discretization/HGRAD/mrhyde.gold     basis values/gradients at Gauss points
   regression/thermal/*/mrhyde.gold                L2 error norms of manufactured solutions
-and writes tests/golden/hgrad_kat.json and tests/golden/thermal_gold.json.
+  regression/navierstokes/channel/mrhyde.gold     L2 error norms of the channel flow
+and writes tests/golden/hgrad_kat.json, thermal_gold.json and ns_gold.json.
 Only numbers (inputs and expected outputs) are stored; the deck parameters of
 each case are transcribed next to its gold values.
 
@@ -72,6 +73,14 @@ CASES = {
 }
 
 
+# navier stokes: regression/navierstokes/channel (Q1-Q1, PSPG, body force 1 in x)
+NS_CASES = {
+    "navierstokes/channel": dict(dim=2, order=1, quadrature=2, n=[50, 10], lo=[0.0, 0.0], hi=[5.0, 1.0],
+                                 vel_dbc_sides=12, pspg=True, supg=False, force=[1.0, 0.0],
+                                 rho=1.0, mu=1.0),
+}
+
+
 def main(ref: Path):
     kat = parse_hgrad(ref / "regression/discretization/HGRAD/mrhyde.gold")
     (HERE / "hgrad_kat.json").write_text(json.dumps(kat, indent=1))
@@ -82,7 +91,13 @@ def main(ref: Path):
     gold["_note"] = ("amplitudes of 'source' are in units of pi^2 (source = amp*pi^2*prod sin(w*pi*x)); "
                      "'true' amplitudes are plain; frequencies are in units of pi")
     (HERE / "thermal_gold.json").write_text(json.dumps(gold, indent=1))
-    print("wrote", HERE / "hgrad_kat.json", HERE / "thermal_gold.json")
+    ns = {}
+    for name, deck in NS_CASES.items():
+        ns[name] = dict(deck, gold=norms(ref / "regression" / name / "mrhyde.gold"))
+    ns["_note"] = ("velocity Dirichlet sides as a mask (left 1, right 2, bottom 4, top 8); true solution "
+                   "ux = 0.5*y*(1-y), uy = pr = 0; density and viscosity at their defaults (1)")
+    (HERE / "ns_gold.json").write_text(json.dumps(ns, indent=1))
+    print("wrote", HERE / "hgrad_kat.json", HERE / "thermal_gold.json", HERE / "ns_gold.json")
 
 
 if __name__ == "__main__":

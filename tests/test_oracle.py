@@ -198,3 +198,79 @@ def test_newton_nonlinear_quadratic_convergence():
     # residual falls super-linearly once in the basin
     assert norms[-1] < 1e-10 * norms[0]
     assert norms[3] < norms[2] ** 1.5 / max(norms[0], 1) * 10 or norms[3] < 1e-12
+
+
+# --- navier stokes ----------------------------------------------------------------------
+from conftest import field_l2_errors, ns_channel_problem  # noqa: E402
+from mrhyde_amd.abi import MHX_PHYS_NAVIER_STOKES  # noqa: E402
+
+NS_GOLD = json.load(open(os.path.join(GOLDEN, "ns_gold.json")))
+CHANNEL_EXACT = [lambda x: 0.5 * x[:, 1] * (1 - x[:, 1])] + [lambda x: 0.0 * x[:, 0]] * 3
+
+
+def test_ns_channel_gold():
+    """regression/navierstokes/channel: 2-D Q1-Q1 PSPG channel, L2 errors of
+    ux, uy and pr reproduced to the printed digits."""
+    deck = NS_GOLD["navierstokes/channel"]
+    o = Oracle(ns_channel_problem(deck))
+    u = o.setDirichlet(np.zeros(o.n_dof))
+    u, st = o.nonlinearSolver(u, nopts(max_iters=10, nl_tol=1e-10, solver=MHX_KRYLOV_GMRES, tol=1e-13))
+    e = field_l2_errors(o, u, CHANNEL_EXACT)
+    gold = deck["gold"]
+    for f, name in enumerate(["ux", "uy", "pr"]):
+        assert abs(e[f] - gold[name]) <= 5e-6 * gold[name], (name, e[f], gold[name])
+
+
+def test_ns_channel_extruded_3d():
+    """The same channel extruded in z (2 cubes of 0.1 deep, natural conditions on
+    the z faces, uz = 0 there): the 3-D discrete solution is the 2-D one, so the L2
+    errors are the 2-D gold values times sqrt(depth) (element size, and so tau,
+    is the same 0.1 in both)."""
+    deck = NS_GOLD["navierstokes/channel"]
+    o = Oracle(ns_channel_problem(deck, nz=2, depth=0.2))
+    u = o.setDirichlet(np.zeros(o.n_dof))
+    u, st = o.nonlinearSolver(u, nopts(max_iters=10, nl_tol=1e-10, solver=MHX_KRYLOV_GMRES, tol=1e-13))
+    e = field_l2_errors(o, u, CHANNEL_EXACT)
+    gold = deck["gold"]
+    s = np.sqrt(0.2)
+    for f, name in [(0, "ux"), (1, "uy"), (3, "pr")]:
+        assert abs(e[f] - s * gold[name]) <= 5e-6 * s * gold[name], (name, e[f], s * gold[name])
+    assert e[2] < 1e-10  # uz
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_ns_jacobian_vs_fd(dim):
+    """SUPG + PSPG element Jacobian (32-slot duals in 3-D) against central
+    differences of the residual on a perturbed mesh with lid, no-slip and a
+    pinned pressure."""
+    rng = np.random.default_rng(31 + dim)
+    prob = Problem(dim=dim, order=1, quadrature=2, n=(2,) * dim, physics=MHX_PHYS_NAVIER_STOKES,
+                   rho=1.3, mu=0.05, supg=True, pspg=True, force=(0.4, -0.2, 0.3), dbc_sides=1 | 4,
+                   lid_sides=8, lid_u=(1.0, 0.0, 0.0), pin_pressure=True, perturb=0.2, perturb_seed=3)
+    o = Oracle(prob)
+    u = o.setDirichlet(rng.standard_normal(o.n_dof))
+    res, jac = o.assembleJacRes(u)
+    rp, ci = o.crs_pattern()
+    J = to_scipy(rp, ci, jac).toarray()
+    mask, vals = o.dirichlet()
+    h = 1e-6
+    free = mask == 0
+    for j in range(o.n_dof):
+        e = np.zeros(o.n_dof)
+        e[j] = h
+        col = -(o.assembleRes(u + e) - o.assembleRes(u - e)) / (2 * h)
+        np.testing.assert_allclose(J[free, j], col[free], rtol=1e-6, atol=1e-7 * np.abs(J).max())
+    fixed = np.where(mask == 1)[0]
+    np.testing.assert_array_equal(J[fixed][:, fixed], np.eye(len(fixed)))
+    assert np.all(res[fixed] == 0.0)
+    # lid and no-slip values; pressure pinned at node 0 (DOF dim)
+    assert mask[dim] == 1 and vals[dim] == 0.0
+    assert set(np.round(vals[fixed], 12)) <= {0.0, 1.0}
+
+
+@pytest.mark.parametrize("dim,n", [(2, (4, 3)), (3, (3, 2, 2))])
+def test_ns_crs_nnz_closed_form(dim, n):
+    o = Oracle(Problem(dim=dim, order=1, quadrature=2, n=n, physics=MHX_PHYS_NAVIER_STOKES))
+    nf = dim + 1
+    assert o.n_dof == nf * int(np.prod([k + 1 for k in n]))
+    assert o.nnz == nf * nf * int(np.prod([3 * k + 1 for k in n]))

@@ -9,7 +9,8 @@ import numpy as np
 import pytest
 
 from conftest import GOLDEN, assert_entries_close, spmv_np, to_scipy
-from mrhyde_amd.abi import MHX_KRYLOV_CG, MHX_KRYLOV_GMRES, KrylovOpts, NewtonOpts, Problem
+from mrhyde_amd.abi import (MHX_KRYLOV_CG, MHX_KRYLOV_GMRES, MHX_PHYS_NAVIER_STOKES, MHX_PREC_JACOBI, KrylovOpts,
+                            NewtonOpts, Problem)
 
 pytestmark = pytest.mark.gpu
 PI = np.pi
@@ -41,6 +42,17 @@ def make_configs():
                                            modes=[(2.0, (PI, PI, PI)), (0.5, (2 * PI, 0, 3 * PI))],
                                            perturb=0.25, perturb_seed=9)
     cfg["3d_q2_quad2"] = Problem(dim=3, order=2, quadrature=2, n=(3, 2, 2), k0=lognormal(12, 0.5, 8))
+    # navier stokes, Q1-Q1
+    cfg["2d_ns_channel_pspg"] = Problem(dim=2, order=1, quadrature=2, n=(10, 4), hi=(5.0, 1.0),
+                                        physics=MHX_PHYS_NAVIER_STOKES, dbc_sides=4 | 8, pspg=True,
+                                        force=(1.0, 0.0))
+    cfg["2d_ns_supg_perturbed"] = Problem(dim=2, order=1, quadrature=2, n=(6, 5), physics=MHX_PHYS_NAVIER_STOKES,
+                                          rho=1.2, mu=0.02, supg=True, pspg=True, dbc_sides=1 | 4, lid_sides=8,
+                                          lid_u=(1.0, 0.0), comp_sides=(0, 2), pin_pressure=True,
+                                          force=(0.3, -0.1), perturb=0.25, perturb_seed=4)
+    cfg["3d_ns_lid_cavity"] = Problem(dim=3, order=1, quadrature=2, n=(5, 4, 3), physics=MHX_PHYS_NAVIER_STOKES,
+                                      mu=0.01, supg=True, pspg=True, dbc_sides=1 | 2 | 4 | 16 | 32, lid_sides=8,
+                                      lid_u=(1.0, 0.0, 0.0), pin_pressure=True, perturb=0.2, perturb_seed=6)
     return cfg
 
 
@@ -142,11 +154,58 @@ def test_linear_solve_parity(pairs, name, solver):
     np.testing.assert_allclose(y, spmv_np(rp, ci, jo, x), rtol=1e-13, atol=1e-14 * np.abs(y).max())
 
 
+@pytest.mark.parametrize("name,solver", [("2d_q1_poisson_c1shape", "cg"), ("3d_q1_lognormal", "cg"),
+                                         ("3d_q2_c2shape", "gmres"), ("2d_q4_quad8", "gmres")])
+def test_mixed_precision_solve(pairs, name, solver):
+    """FP32 Jacobian, preconditioner and inner Krylov inside FP64 iterative
+    refinement: the FP64 true residual reaches 1e-10 and the result agrees
+    with the all-FP64 solve."""
+    from mrhyde_amd import krylov_opts
+
+    g, o = pairs[name]
+    u = o.setDirichlet(np.zeros(o.n_dof))
+    ro, jo = o.assembleJacRes(u)
+    rp, ci = o.crs_pattern()
+    print(f"[mixed] {name} {solver} n_dof={g.n_dof}", flush=True)
+    x32, st = g.linearSolver(jo, ro, opts=krylov_opts(solver=solver, tol=1e-10, max_iters=5000, precond="jacobi-fp32"))
+    x64, st64 = g.linearSolver(jo, ro, opts=krylov_opts(solver=solver, tol=1e-10, max_iters=5000))
+    rel = np.linalg.norm(ro - spmv_np(rp, ci, jo, x32)) / np.linalg.norm(ro)
+    assert st["rel_res"] <= 1e-10 and rel <= 1.5e-10, (st, rel)
+    assert st["refinements"] >= 2, st  # FP32 alone cannot reach 1e-10
+    # error against the all-FP64 solve, bounded by the conditioning times the tolerance
+    np.testing.assert_allclose(x32, x64, rtol=0, atol=1e-7 * np.abs(x64).max())
+
+
+@pytest.mark.parametrize("precond", ["none", "jacobi", "block jacobi"])
+def test_preconditioner_variants(pairs, precond):
+    from mrhyde_amd import krylov_opts
+
+    g, o = pairs["2d_q2_nonlinear_perturbed"]
+    u = random_state(o, 5)
+    ro, jo = o.assembleJacRes(u)
+    rp, ci = o.crs_pattern()
+    x, st = g.linearSolver(jo, ro, opts=krylov_opts(solver="gmres", tol=1e-11, max_iters=5000, precond=precond))
+    rel = np.linalg.norm(ro - spmv_np(rp, ci, jo, x)) / np.linalg.norm(ro)
+    assert st["rel_res"] <= 1e-11 and rel <= 2e-11, (precond, st, rel)
+
+
+def test_unknown_preconditioner_is_rejected(pairs):
+    from mrhyde_amd import MhxError, krylov_opts
+
+    g, o = pairs["2d_q1_poisson_c1shape"]
+    ro, jo = o.assembleJacRes(o.setDirichlet(np.zeros(o.n_dof)))
+    opts = krylov_opts()
+    opts.precond = 7
+    with pytest.raises(MhxError):
+        g.linearSolver(jo, ro, opts=opts)
+
+
 def nopts(max_iters, nl_tol, solver, tol=1e-12, backtracking=1):
     o = NewtonOpts()
     o.max_iters, o.nl_tol, o.backtracking = max_iters, nl_tol, backtracking
     o.linear = KrylovOpts()
     o.linear.solver, o.linear.tol, o.linear.max_iters, o.linear.restart = solver, tol, 20000, 50
+    o.linear.precond = MHX_PREC_JACOBI
     return o
 
 
@@ -179,6 +238,66 @@ def test_gold_l2_on_gpu(case):
     assert float(f"{l2:.6g}") == gd["gold"]["T"]
     if "grad(T)" in gd["gold"]:
         assert float(f"{l2g:.6g}") == gd["gold"]["grad(T)"]
+
+
+NS_GOLD = json.load(open(os.path.join(GOLDEN, "ns_gold.json")))
+CHANNEL_EXACT = [lambda x: 0.5 * x[:, 1] * (1 - x[:, 1])] + [lambda x: 0.0 * x[:, 0]] * 3
+
+
+@pytest.mark.parametrize("extrude", [0, 2])
+def test_ns_channel_gold_on_gpu(extrude):
+    """regression/navierstokes/channel solved on the GPU (Newton, GMRES with
+    nodal block Jacobi): L2 errors of ux, uy, pr equal the gold values (2-D),
+    or the gold values times sqrt(depth) for the extruded 3-D channel."""
+    from conftest import field_l2_errors, ns_channel_problem
+    from mrhyde_amd import Mhx
+    from oracle import Oracle
+
+    deck = NS_GOLD["navierstokes/channel"]
+    p = ns_channel_problem(deck, nz=extrude, depth=0.1 * extrude)
+    g = Mhx(p)
+    print(f"[ns] channel extrude={extrude} n_dof={g.n_dof}", flush=True)
+    u = g.setDirichlet(np.zeros(g.n_dof))
+    u, st = g.nonlinearSolver(u, max_iters=10, nl_tol=1e-10, solver="gmres", tol=1e-13, precond="block jacobi",
+                              linear_max_iters=20000)
+    e = field_l2_errors(Oracle(p), u, CHANNEL_EXACT)
+    s = np.sqrt(0.1 * extrude) if extrude else 1.0
+    fields = [(0, "ux"), (1, "uy"), (p.dim, "pr")]
+    for f, name in fields:
+        ref = s * deck["gold"][name]
+        assert abs(e[f] - ref) <= 5e-6 * ref, (name, e[f], ref)
+
+
+def test_ns_newton_parity(pairs):
+    """3-D lid-driven cavity (SUPG + PSPG, perturbed mesh): GPU Newton with
+    GMRES + block Jacobi against the oracle's Newton."""
+    g, o = pairs["3d_ns_lid_cavity"]
+    u0 = o.setDirichlet(np.zeros(o.n_dof))
+    ug, sg = g.nonlinearSolver(u0.copy(), max_iters=8, nl_tol=1e-10, solver="gmres", tol=1e-12,
+                               precond="block jacobi", linear_max_iters=20000)
+    uo, so = o.nonlinearSolver(u0.copy(), nopts(8, 1e-10, MHX_KRYLOV_GMRES))
+    np.testing.assert_allclose(ug, uo, rtol=0, atol=1e-9 * np.abs(uo).max())
+    assert np.abs(o.assembleRes(ug)).max() <= 1e-9 * sg["res_norm0"]
+
+
+def test_ns_mixed_precision_newton():
+    """The mixed-precision solve path on a 3-D cavity: FP32 Jacobian, block-
+    Jacobi preconditioner and inner GMRES inside FP64 refinement to 1e-10
+    relative residual at every Newton step; the converged state matches the
+    all-FP64 Newton (error stated as the bound below)."""
+    from mrhyde_amd import Mhx, rng
+
+    p = Problem(dim=3, order=1, quadrature=2, n=(8, 8, 8), physics=MHX_PHYS_NAVIER_STOKES, mu=0.01, supg=True,
+                pspg=True, dbc_sides=1 | 2 | 4 | 16 | 32, lid_sides=8, lid_u=(1.0, 0.0, 0.0), pin_pressure=True)
+    g = Mhx(p)
+    u0 = g.setDirichlet(1e-3 * rng(2026, g.n_dof, 2, 0.0, 1.0))
+    kw = dict(max_iters=12, nl_tol=1e-9, solver="gmres", tol=1e-10, linear_max_iters=20000)
+    u32, s32 = g.nonlinearSolver(u0.copy(), precond="block jacobi-fp32", **kw)
+    u64, s64 = g.nonlinearSolver(u0.copy(), precond="block jacobi", **kw)
+    print(f"[ns] mixed: nl {s32['nl_iters']} refinements {s32['refinements']} vs fp64 nl {s64['nl_iters']}", flush=True)
+    assert s32["refinements"] >= 2 * s32["nl_iters"] - 2
+    assert s32["res_norm"] <= 1e-9 * s32["res_norm0"] and s64["res_norm"] <= 1e-9 * s64["res_norm0"]
+    np.testing.assert_allclose(u32, u64, rtol=0, atol=1e-8 * np.abs(u64).max())
 
 
 # --- failure handling, forced from the host (every kernel still exits normally) ----
