@@ -104,6 +104,146 @@ def krylov_bench(g, iters: int):
     return out
 
 
+def ns_bench(n: int, steps: int, newton: bool = True):
+    """Steady Navier-Stokes, Q1-Q1 with SUPG/PSPG: lid-driven cavity on [0,1]^3
+    with n^3 hexes, Re = 100 (viscosity 0.01, lid speed 1 on y = 1, no-slip
+    elsewhere, pressure pinned at the origin), initial velocity N(0, 1e-3)
+    seeded.  J+R assembly rate and its roofline; then Newton with GMRES(50) +
+    nodal block Jacobi to 1e-10 per step, FP32 inner solve in FP64
+    refinement, against the all-FP64 Newton (difference stated)."""
+    from mrhyde_amd import Mhx, rng
+    from mrhyde_amd.abi import MHX_BUF_U, MHX_PHYS_NAVIER_STOKES, Problem
+
+    p = Problem(dim=3, order=1, quadrature=2, n=(n, n, n), physics=MHX_PHYS_NAVIER_STOKES, mu=0.01, supg=True,
+                pspg=True, dbc_sides=1 | 2 | 4 | 16 | 32, lid_sides=8, lid_u=(1.0, 0.0, 0.0), pin_pressure=True)
+    g = Mhx(p)
+    u0 = g.setDirichlet(1e-3 * rng(2027, g.n_dof, 2, 0.0, 1.0))
+    g.state_upload(MHX_BUF_U, u0)
+    for _ in range(2):
+        g.assemble_resident()
+    g.synchronize()
+    g.profile(True)
+    g.profile_reset()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        g.assemble_resident()
+    g.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    prof = g.profile_read()
+    g.profile(False)
+    k_ms, k_cnt = prof.get("k_ns<jacres>", (float("nan"), 0))
+    per_launch = k_ms / max(k_cnt, 1)
+    n_vert = (n + 1) ** 3
+    B = b_asm(g.n_elem, 32, 0, n_vert, 3)
+    lpl = k_cnt / steps
+    out = {"workload": f"3-D lid-driven cavity Re=100, {n}^3 Q1-Q1 hexes, SUPG+PSPG, 4 unknowns per node",
+           "n_dof": g.n_dof, "nnz": g.nnz, "assembly_ms": round(dt * 1e3, 4),
+           "assembly_MDOFps": round(g.n_dof / dt / 1e6, 2),
+           "roofline": {"kernel": "k_ns<jacres>", "B_asm": B, "launches_per_step": lpl,
+                        "mean_launch_ms": round(per_launch, 5),
+                        "achieved_GBps": round(B / lpl / (per_launch * 1e-3) / 1e9, 1),
+                        "frac": round(B / lpl / (per_launch * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}}
+    if newton:
+        kw = dict(max_iters=10, nl_tol=1e-8, solver="gmres", restart=50, tol=1e-10, linear_max_iters=5000)
+        res = {}
+        for name, prec in (("fp32-refined", "block jacobi-fp32"), ("fp64", "block jacobi")):
+            t0 = time.perf_counter()
+            u, st = g.nonlinearSolver(u0.copy(), precond=prec, **kw)
+            res[name] = (u, st, time.perf_counter() - t0)
+        u32, s32, t32 = res["fp32-refined"]
+        u64, s64, t64 = res["fp64"]
+        out["newton"] = {
+            name: {"wall_s": round(t, 3), "nl_iters": st["nl_iters"], "krylov_iters": st["iters"],
+                   "refinements": st["refinements"], "res_norm0": st["res_norm0"], "res_norm": st["res_norm"],
+                   "last_linear_rel_res": st["rel_res"]}
+            for name, (u, st, t) in res.items()}
+        out["newton"]["max_abs_diff_fp32r_vs_fp64"] = float(np.abs(u32 - u64).max())
+        out["newton"]["max_abs_u"] = float(np.abs(u64).max())
+    g.close()
+    return out
+
+
+def c5_requests(count: int, seed: int):
+    """Thermal solve requests: with probability 1/2 a 2-D Q1 mesh of n^2
+    elements, n log-uniform on [32, 256], else a 3-D mesh of n^3 elements, n
+    log-uniform on [16, 64], Q1 or Q2 with probability 1/2; k = exp(0.5 z) per
+    element; source sum_{m=1..M} a_m prod_d sin(w_md pi x_d), M ~ U{1..4},
+    a_m ~ U[0,1], w ~ U{1..4}; T = 0 on the boundary."""
+    from mrhyde_amd.abi import Problem
+
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        if rng.random() < 0.5:
+            dim, order, n = 2, 1, int(round(np.exp(rng.uniform(np.log(32), np.log(256)))))
+        else:
+            dim, order, n = 3, int(rng.integers(1, 3)), int(round(np.exp(rng.uniform(np.log(16), np.log(64)))))
+        ne = n ** dim
+        modes = [(float(rng.uniform(0, 1)), [np.pi * int(rng.integers(1, 5)) for _ in range(dim)])
+                 for _ in range(int(rng.integers(1, 5)))]
+        out.append(Problem(dim=dim, order=order, quadrature=2 * order, n=(n,) * dim,
+                           k0=np.exp(0.5 * rng.standard_normal(ne)), modes=modes))
+    return out
+
+
+def serving_bench(n_req: int = 240, rate: float = 200.0, seed: int = 5, per_call_sample: int = 12):
+    """Poisson arrivals at `rate` of C5 requests into the server (8 concurrent
+    requests, resident contexts reused), Jacobi-CG to 1e-10; latency from
+    submit to result.  Then the same requests all at once (capacity) in FP64
+    and with the FP32-inner / FP64-refined CG, and the plain per-call path
+    (a fresh context per request, sequential) on a sample."""
+    from mrhyde_amd import Mhx, Server
+
+    probs = c5_requests(n_req, seed)
+    rng = np.random.default_rng(seed + 1)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n_req))
+    out = {"workload": f"{n_req} thermal solve requests (2-D Q1 32^2..256^2, 3-D Q1/Q2 16^3..64^3, log-uniform), "
+                       f"Jacobi-CG to 1e-10", "arrival_rate": rate,
+           "total_dof": int(sum(p.n_dof for p in probs))}
+
+    def run(srv, timed_arrivals):
+        t0 = time.perf_counter()
+        tickets = []
+        for p, ta in zip(probs, timed_arrivals):
+            dt = t0 + ta - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            tickets.append(srv.submit(p))
+        res = [srv.poll(t) for t in tickets]
+        wall = time.perf_counter() - t0
+        lat = np.array([r[2] for r in res])
+        rel = max(r[1]["rel_res"] for r in res)
+        return {"wall_s": round(wall, 3), "solves_per_s": round(n_req / wall, 1),
+                "p50_ms": round(float(np.percentile(lat, 50)), 2), "p99_ms": round(float(np.percentile(lat, 99)), 2),
+                "max_linear_rel_res": rel, "server": srv.info()}
+
+    srv = Server(max_active=8, max_cached=48, solver="cg", precond="jacobi", tol=1e-10)
+    out["poisson"] = run(srv, arrivals)
+    srv.close()
+    for name, prec in (("saturated_fp64", "jacobi"), ("saturated_fp32_refined", "jacobi-fp32")):
+        srv = Server(max_active=8, max_cached=48, solver="cg", precond=prec, tol=1e-10)
+        out[name] = run(srv, np.zeros(n_req))
+        srv.close()
+    idx = np.linspace(0, n_req - 1, per_call_sample).astype(int)
+    t_all = 0.0
+    for i in idx:
+        t0 = time.perf_counter()
+        g = Mhx(probs[i])
+        u = g.setDirichlet(np.zeros(g.n_dof))
+        g.nonlinearSolver(u, max_iters=2, nl_tol=1e-8, solver="cg", tol=1e-10, linear_max_iters=20000)
+        g.close()
+        t_all += time.perf_counter() - t0
+    # per-call time scaled to the whole request set by DOF count
+    dof_sample = sum(probs[i].n_dof for i in idx)
+    per_call_wall = t_all * out["total_dof"] / dof_sample
+    out["per_call"] = {"sample": f"{per_call_sample} requests evenly spaced over the set, fresh context each, "
+                                 "sequential; time scaled to the whole set by DOF count",
+                       "solves_per_s": round(n_req / per_call_wall, 1)}
+    best = max(out["saturated_fp64"]["solves_per_s"], out["saturated_fp32_refined"]["solves_per_s"])
+    out["speedup_served_vs_per_call"] = round(best / out["per_call"]["solves_per_s"], 2)
+    return out
+
+
 def subgrid_bench(steps: int, per_call_sample: int = 16):
     """4096 independent Q1 thermal subproblems (n in {4,6,8,12,16}, lognormal
     k with sigma 1, Dirichlet data U[-1,1]) assembled and solved to 1e-10 in
@@ -176,6 +316,9 @@ def main():
     ap.add_argument("--probe-gib", type=float, default=1.0, help="size of the plain-copy bandwidth probe")
     ap.add_argument("--skip-subgrid", action="store_true", help="skip the batched-subgrid measurement")
     ap.add_argument("--krylov-iters", type=int, default=100, help="GMRES iterations timed (0 = skip)")
+    ap.add_argument("--ns-n", type=int, default=64, help="Navier-Stokes cavity elements per axis (0 = skip)")
+    ap.add_argument("--ns-no-newton", action="store_true", help="Navier-Stokes: assembly only")
+    ap.add_argument("--serve-requests", type=int, default=240, help="serving requests (0 = skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
@@ -236,12 +379,21 @@ def main():
         if capi.lib.mhx_copy_probe(nd, 5, C.byref(ms)) == 0:
             probe = round(16.0 * nd / (ms.value * 1e-3) / 1e9, 1)
 
-    cpu = None
-    if not args.no_cpu_baseline:
-        cpu = cpu_baseline(n)
+    def step(name, fn):
+        t0 = time.perf_counter()
+        print(f"[bench] {name} ...", file=sys.stderr, flush=True)
+        r = fn()
+        print(f"[bench] {name} done in {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
+        return r
 
-    krylov = krylov_bench(g, args.krylov_iters) if args.krylov_iters > 0 else None
-    subgrid = None if args.skip_subgrid else subgrid_bench(max(3, min(args.steps, 10)))
+    print(f"[bench] assembly {ms_step:.3f} ms/step, {mdofs:.1f} MDOF/s", file=sys.stderr, flush=True)
+    cpu = None if args.no_cpu_baseline else step("cpu baseline", lambda: cpu_baseline(n))
+    krylov = step("krylov", lambda: krylov_bench(g, args.krylov_iters)) if args.krylov_iters > 0 else None
+    g.close()
+    steps_small = max(3, min(args.steps, 10))
+    subgrid = None if args.skip_subgrid else step("subgrid", lambda: subgrid_bench(steps_small))
+    ns = step("navier stokes", lambda: ns_bench(args.ns_n, steps_small, not args.ns_no_newton)) if args.ns_n > 0 else None
+    serving = step("serving", lambda: serving_bench(args.serve_requests)) if args.serve_requests > 0 else None
 
     out = {
         "metric": "assembly MDOF/s (% HBM peak); subgrid solves/s; p50/p99 solve latency @1e-10",
@@ -269,6 +421,8 @@ def main():
         "phases_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in prof.items()},
         "krylov": krylov,
         "subgrid": subgrid,
+        "navier_stokes": ns,
+        "serving": serving,
     }
     print(json.dumps(out), flush=True)
 

@@ -49,6 +49,7 @@ extern "C" {
 #define MHX_SIDE_ALL  63u   /* "all boundaries" */
 
 #define MHX_MAX_MODES 8
+#define MHX_MAX_NQ_MODES 4
 
 /*
  * Problem description: the subset of a MrHyDE input deck this path needs.
@@ -82,6 +83,16 @@ typedef struct mhx_problem_desc {
   int32_t n_modes;
   double  mode_amp[MHX_MAX_MODES];
   double  mode_freq[MHX_MAX_MODES][3];
+
+  /* Neumann conditions (thermal boundaryResidual, "Neumann",
+   * src/physics/thermal.cpp:210-222): on the sides in neumann_sides the
+   * outward flux g = q(x) . n enters as res += int_side g phi, with
+   * q_c(x) = sum_m nq_amp[m][c] * prod_d sin(nq_freq[m][d] x_d + nq_phase[m][d]). */
+  uint32_t neumann_sides;
+  int32_t  n_nq_modes;    /* <= MHX_MAX_NQ_MODES */
+  double  nq_amp[MHX_MAX_NQ_MODES][3];
+  double  nq_freq[MHX_MAX_NQ_MODES][3];
+  double  nq_phase[MHX_MAX_NQ_MODES][3];
 
   /* strong Dirichlet data (reference: "Dirichlet conditions", use strong DBCs) */
   uint32_t dbc_sides;     /* MHX_SIDE_* mask */
@@ -300,6 +311,52 @@ int  mhx_batch_results(mhx_batch* b, double* x, int32_t* iters, double* relres, 
 int  mhx_batch_info(const mhx_batch* b, int64_t* total_nodes, int32_t* wg_size, double* last_ms);
 int  mhx_batch_set_wg(mhx_batch* b, int32_t wg_size);
 void mhx_batch_destroy(mhx_batch* b);
+
+/* ---- sizes and coefficient updates -------------------------------------
+ * n_dof, nnz and the device bytes a context needs for assembly and one
+ * Krylov solve with the given options (opts may be NULL: CG), without
+ * building it.  No reference counterpart (admission control of the server). */
+int  mhx_footprint(const mhx_problem_desc* desc, const mhx_krylov_opts* opts, int64_t* n_dof, int64_t* nnz,
+                   int64_t* device_bytes);
+/* New coefficients on an existing context: element k0 / source, source modes,
+ * Dirichlet values, Navier-Stokes parameters.  Mesh, discretisation, physics
+ * and Dirichlet sides must equal the context's (MHX_ERR_ARG otherwise); the
+ * mesh, CRS graph and solver buffers stay resident.  The role of re-running
+ * a sample with new parameters in the reference's UQ loop
+ * (AnalysisManager::forwardModel, src/managers/analysis/analysisManager_solve.hpp). */
+int  mhx_ctx_update(mhx_ctx* ctx, const mhx_problem_desc* desc);
+
+/* ---- serving: continuous batching of independent solves -----------------
+ * Each request is one steady solve (mhx_nonlinear_solve with the server's
+ * Newton options, from u = Dirichlet data, 0 elsewhere).  Up to max_active
+ * requests run at once, each on its own host worker and HIP stream; a queued
+ * request is admitted (first fit) when a worker is free and its footprint
+ * (mhx_footprint) fits hbm_budget next to the contexts held; an idle context
+ * with the same mesh is reused (max_cached idle contexts are kept).  Results
+ * do not depend on the mix: reductions run in a fixed order per request.
+ * No counterpart in the reference (its UQ loop runs samples one by one,
+ * src/managers/analysis/analysisManager_solve.hpp). */
+typedef struct mhx_server mhx_server;
+typedef struct mhx_server_desc {
+  int64_t hbm_budget;       /* bytes of device memory for contexts; 0 = half the free memory */
+  int32_t max_active;       /* concurrent requests (default 8) */
+  int32_t max_cached;       /* idle contexts kept for reuse (0 = none) */
+  mhx_newton_opts newton;   /* per request; zeros -> 2 Newton steps, nl_tol 1e-8, CG + linear.precond to 1e-10 */
+  int32_t reserved[4];
+} mhx_server_desc;
+int  mhx_server_create(const mhx_server_desc* desc, mhx_server** out);
+/* Copies the description and its arrays; never blocks on the device.
+ * MHX_ERR_NOMEM when the request alone exceeds hbm_budget. */
+int  mhx_server_submit(mhx_server* s, const mhx_problem_desc* desc, int64_t* ticket);
+/* 0: still queued/running (wait = 0); 1: done, x[n_dof] (host, may be NULL),
+ * stats and the latency from submit to completion filled in and the ticket
+ * released; < 0: the request failed (message: mhx_server_last_error). */
+int  mhx_server_poll(mhx_server* s, int64_t ticket, int32_t wait, double* x, mhx_solve_stats* stats,
+                     double* latency_ms);
+/* done, active, queued, bytes held, contexts reused, contexts created, idle contexts, budget */
+int  mhx_server_info(mhx_server* s, int64_t* out8);
+const char* mhx_server_last_error(mhx_server* s);
+void mhx_server_destroy(mhx_server* s);
 
 /* Plain device copy y = x of n_doubles FP64 values, repeated reps times on a
  * private stream; returns the mean milliseconds per copy.  A calibration

@@ -30,6 +30,7 @@ MHX_RES, MHX_JAC, MHX_ACCUMULATE, MHX_DEVICE_PTRS = 1, 2, 4, 8
 MHX_KRYLOV_CG, MHX_KRYLOV_GMRES = 1, 2
 MHX_PREC_NONE, MHX_PREC_JACOBI, MHX_PREC_BLOCK_JACOBI, MHX_PREC_FP32 = 0, 1, 2, 0x100
 MHX_MAX_MODES = 8
+MHX_MAX_NQ_MODES = 4
 MHX_RNG_UNIFORM, MHX_RNG_LOGNORMAL, MHX_RNG_NORMAL = 0, 1, 2
 MHX_BUF_U, MHX_BUF_RES, MHX_BUF_JAC, MHX_BUF_DU, MHX_BUF_AUX = 0, 1, 2, 3, 4
 
@@ -48,6 +49,9 @@ class ProblemDesc(C.Structure):
         ("n_modes", C.c_int32),
         ("mode_amp", C.c_double * MHX_MAX_MODES),
         ("mode_freq", (C.c_double * 3) * MHX_MAX_MODES),
+        ("neumann_sides", C.c_uint32), ("n_nq_modes", C.c_int32),
+        ("nq_amp", (C.c_double * 3) * MHX_MAX_NQ_MODES), ("nq_freq", (C.c_double * 3) * MHX_MAX_NQ_MODES),
+        ("nq_phase", (C.c_double * 3) * MHX_MAX_NQ_MODES),
         ("dbc_sides", C.c_uint32), ("dbc_const", C.c_double), ("dbc_values", _dp),
         ("ns_rho", C.c_double), ("ns_mu", C.c_double), ("ns_supg", C.c_int32), ("ns_pspg", C.c_int32),
         ("ns_lid_sides", C.c_uint32), ("ns_lid_u", C.c_double * 3),
@@ -79,6 +83,13 @@ class SolveStats(C.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class ServerDesc(C.Structure):
+    _fields_ = [
+        ("hbm_budget", C.c_int64), ("max_active", C.c_int32), ("max_cached", C.c_int32),
+        ("newton", NewtonOpts), ("reserved", C.c_int32 * 4),
+    ]
 
 
 MHX_BATCH_FP64, MHX_BATCH_FP32_REFINED = 0, 1
@@ -145,6 +156,10 @@ class Problem:
     diff_u2: float = 0.0
     src_elem: Optional[np.ndarray] = None
     modes: Sequence = ()
+    # Neumann sides and flux field q(x): terms (amp[3], freq[3], phase[3]),
+    # q_c = sum amp_c prod_d sin(freq_d x_d + phase_d); g = q . n
+    neumann_sides: int = 0
+    neumann_flux: Sequence = ()
     dbc_sides: int = MHX_SIDE_ALL
     dbc_const: float = 0.0
     dbc_values: Optional[np.ndarray] = None
@@ -213,6 +228,14 @@ class Problem:
             fr = list(freq) + [0.0] * (3 - len(freq))
             for a in range(3):
                 d.mode_freq[m][a] = float(fr[a])
+        if len(self.neumann_flux) > MHX_MAX_NQ_MODES:
+            raise ValueError("too many Neumann flux terms")
+        d.neumann_sides, d.n_nq_modes = int(self.neumann_sides), len(self.neumann_flux)
+        for m, (amp, freq, phase) in enumerate(self.neumann_flux):
+            for a in range(3):
+                d.nq_amp[m][a] = float((list(amp) + [0.0] * 3)[a])
+                d.nq_freq[m][a] = float((list(freq) + [0.0] * 3)[a])
+                d.nq_phase[m][a] = float((list(phase) + [0.0] * 3)[a])
         d.dbc_sides, d.dbc_const = int(self.dbc_sides), float(self.dbc_const)
         d.dbc_values = arr(self.dbc_values, self.n_dof)
         d.ns_rho, d.ns_mu = float(self.rho), float(self.mu)

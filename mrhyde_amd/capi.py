@@ -9,7 +9,7 @@ import ctypes as C
 from pathlib import Path
 
 from .abi import *  # noqa: F401,F403
-from .abi import BatchDesc, KrylovOpts, NewtonOpts, ProblemDesc, SolveStats
+from .abi import BatchDesc, KrylovOpts, NewtonOpts, ProblemDesc, ServerDesc, SolveStats
 
 LIB_PATH = Path(__file__).resolve().parent / "libmhx.so"
 
@@ -52,6 +52,15 @@ def _load() -> C.CDLL:
         "mhx_state_upload": (C.c_int, [vp, i32, vp, i64]),
         "mhx_state_download": (C.c_int, [vp, i32, vp, i64]),
         "mhx_rng_fill": (C.c_int, [C.c_uint64, i64, i32, C.c_double, C.c_double, vp]),
+        "mhx_footprint": (C.c_int, [C.POINTER(ProblemDesc), C.POINTER(KrylovOpts), C.POINTER(i64), C.POINTER(i64),
+                                    C.POINTER(i64)]),
+        "mhx_ctx_update": (C.c_int, [vp, C.POINTER(ProblemDesc)]),
+        "mhx_server_create": (C.c_int, [C.POINTER(ServerDesc), C.POINTER(vp)]),
+        "mhx_server_submit": (C.c_int, [vp, C.POINTER(ProblemDesc), C.POINTER(i64)]),
+        "mhx_server_poll": (C.c_int, [vp, i64, i32, vp, C.POINTER(SolveStats), C.POINTER(C.c_double)]),
+        "mhx_server_info": (C.c_int, [vp, vp]),
+        "mhx_server_last_error": (C.c_char_p, [vp]),
+        "mhx_server_destroy": (None, [vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -69,4 +78,6 @@ EXPORTED = [
     "mhx_nonlinear_solve", "mhx_l2_error", "mhx_profile_enable", "mhx_profile_read", "mhx_profile_reset",
     "mhx_synchronize", "mhx_set_stream", "mhx_set_option", "mhx_get_option", "mhx_copy_probe", "mhx_batch_create", "mhx_batch_solve",
     "mhx_batch_results", "mhx_batch_info", "mhx_batch_set_wg", "mhx_batch_destroy","mhx_state_ptr", "mhx_state_upload", "mhx_state_download", "mhx_rng_fill", "mhx_version",
+    "mhx_footprint", "mhx_ctx_update", "mhx_server_create", "mhx_server_submit", "mhx_server_poll",
+    "mhx_server_info", "mhx_server_last_error", "mhx_server_destroy",
 ]

@@ -802,6 +802,7 @@ void assemble_thermal(Ctx& c, const double* u, double* res, double* jac, bool wa
   {
     ScopedPhase ph(c.prof, want_jac ? "AssemblyManager::assembleJacRes" : "AssemblyManager::assembleRes", c.stream);
     f(c, a, want_jac);
+    if (want_res) neumann_residual(c, res);  // boundary groups, after the volume groups
   }
   if (accumulate && want_jac && !c.h_dbc_list.empty()) {
     ScopedPhase ph(c.prof, "AssemblyManager::dofConstraints", c.stream);

@@ -69,6 +69,9 @@ static void validate(const mhx_problem_desc* d) {
   MHX_REQUIRE(d->xmax > d->xmin && d->ymax > d->ymin && (d->dim == 2 || d->zmax > d->zmin), MHX_ERR_ARG,
               "empty box");
   MHX_REQUIRE(d->n_modes >= 0 && d->n_modes <= MHX_MAX_MODES, MHX_ERR_ARG, "n_modes out of range");
+  MHX_REQUIRE(d->n_nq_modes >= 0 && d->n_nq_modes <= MHX_MAX_NQ_MODES, MHX_ERR_ARG, "n_nq_modes out of range");
+  MHX_REQUIRE(d->neumann_sides == 0 || d->physics == MHX_PHYS_THERMAL, MHX_ERR_UNSUPPORTED,
+              "Neumann sides are built for the thermal module only");
   if (d->physics == MHX_PHYS_NAVIER_STOKES) {
     MHX_REQUIRE(d->ns_rho > 0.0 && d->ns_mu > 0.0, MHX_ERR_ARG, "navier stokes: density and viscosity must be > 0");
     MHX_REQUIRE(ns_supported(d->dim, d->order, d->quadrature / 2 + 1), MHX_ERR_UNSUPPORTED,
@@ -600,6 +603,96 @@ int mhx_synchronize(mhx_ctx* ctx) {
     return MHX_ERR_DEVICE;
   }
   return MHX_OK;
+}
+
+// sum over the lattice nodes of one axis of the 1-D row length (the CRS graph
+// is the tensor product of these ranges)
+static int64_t axis_pattern_sum(int p, int nel) {
+  const int nn = p * nel + 1;
+  int64_t s = 0;
+  for (int c = 0; c < nn; ++c) {
+    if (c % p == 0) s += std::min(c + p, nn - 1) - std::max(c - p, 0) + 1;
+    else s += p + 1;
+  }
+  return s;
+}
+
+int mhx_footprint(const mhx_problem_desc* d, const mhx_krylov_opts* opts, int64_t* n_dof, int64_t* nnz,
+                  int64_t* device_bytes) {
+  MHX_GUARD_BEGIN
+  validate(d);
+  const int nf = d->physics == MHX_PHYS_NAVIER_STOKES ? d->dim + 1 : 1;
+  const int nel[3] = {d->nx, d->ny, d->dim == 3 ? d->nz : 1};
+  int64_t N = nf, Z = (int64_t)nf * nf, ne = 1, nv = 1;
+  for (int a = 0; a < d->dim; ++a) {
+    N *= (int64_t)d->order * nel[a] + 1;
+    Z *= axis_pattern_sum(d->order, nel[a]);
+    ne *= nel[a];
+    nv *= nel[a] + 1;
+  }
+  // CRS graph + values; u, res, du, aux, staging, Dirichlet tables; CG
+  // vectors r, z, p, q and the preconditioner; mesh arrays; reduction space
+  int64_t bytes = 12 * Z + 4 * (N + 1) + N * (8 * 4 + 8 * 3 + 8 + 1 + 4) + 8 * 5 * N + 24 * nv + 16 * ne + (1 << 20);
+  if (opts && opts->solver != MHX_KRYLOV_CG) bytes += 8 * N * ((opts->restart > 0 ? opts->restart : 50) + 3);
+  if (opts && (opts->precond & MHX_PREC_FP32)) bytes += 4 * Z + 4 * 8 * N;
+  if (n_dof) *n_dof = N;
+  if (nnz) *nnz = Z;
+  if (device_bytes) *device_bytes = bytes;
+  return MHX_OK;
+  MHX_GUARD_END((mhx_ctx*)nullptr)
+}
+
+int mhx_ctx_update(mhx_ctx* ctx, const mhx_problem_desc* d) {
+  MHX_GUARD_BEGIN
+  MHX_REQUIRE(ctx && d, MHX_ERR_ARG, "null argument");
+  validate(d);
+  const mhx_problem_desc& o = ctx->desc;
+  const bool same = d->dim == o.dim && d->order == o.order && d->quadrature == o.quadrature && d->nx == o.nx &&
+                    d->ny == o.ny && (d->dim == 2 || d->nz == o.nz) && d->xmin == o.xmin && d->xmax == o.xmax &&
+                    d->ymin == o.ymin && d->ymax == o.ymax && d->zmin == o.zmin && d->zmax == o.zmax &&
+                    d->perturb == o.perturb && d->perturb_seed == o.perturb_seed && d->physics == o.physics &&
+                    d->dbc_sides == o.dbc_sides && d->ns_lid_sides == o.ns_lid_sides &&
+                    d->ns_pin_pressure == o.ns_pin_pressure && d->ns_comp_sides[0] == o.ns_comp_sides[0] &&
+                    d->ns_comp_sides[1] == o.ns_comp_sides[1] && d->ns_comp_sides[2] == o.ns_comp_sides[2];
+  MHX_REQUIRE(same, MHX_ERR_ARG, "ctx_update: mesh, discretisation or Dirichlet sides differ from the context's");
+  Ctx& c = *ctx;
+  const BoxMesh& m = c.mesh;
+  auto upload = [&](const double* src, DevBuf<double>& buf) {
+    if (src) {
+      buf.alloc(m.ne);
+      MHX_HIP(hipMemcpyAsync(buf.p, src, m.ne * sizeof(double), hipMemcpyHostToDevice, c.stream));
+    } else {
+      buf.release();
+    }
+  };
+  upload(d->k0, c.k0);
+  upload(d->src_elem, c.src);
+  // Dirichlet values on the (unchanged) Dirichlet DOFs
+  const bool ns = d->physics == MHX_PHYS_NAVIER_STOKES;
+  for (int32_t g : c.h_dbc_list) {
+    if (!ns) {
+      c.h_dbc[g] = d->dbc_values ? d->dbc_values[g] : d->dbc_const;
+    } else {
+      const int f = g % m.nf;
+      const long node = g / m.nf;
+      const long gc[3] = {node % m.nn[0], (node / m.nn[0]) % m.nn[1], node / ((long)m.nn[0] * m.nn[1])};
+      uint32_t on = 0;
+      for (int a = 0; a < m.dim; ++a) {
+        if (gc[a] == 0) on |= 1u << (2 * a);
+        if (gc[a] == m.nn[a] - 1) on |= 1u << (2 * a + 1);
+      }
+      c.h_dbc[g] = (f < m.dim && (on & d->ns_lid_sides)) ? d->ns_lid_u[f] : 0.0;
+    }
+  }
+  MHX_HIP(hipMemcpyAsync(c.dbc_vals.p, c.h_dbc.data(), m.ndof * sizeof(double), hipMemcpyHostToDevice, c.stream));
+  MHX_HIP(hipStreamSynchronize(c.stream));
+  c.desc = *d;
+  c.desc.k0 = nullptr;
+  c.desc.src_elem = nullptr;
+  c.desc.dbc_values = nullptr;
+  c.jac_valid = false;
+  return MHX_OK;
+  MHX_GUARD_END(ctx)
 }
 
 int mhx_set_option(mhx_ctx* ctx, const char* name, int64_t value) {

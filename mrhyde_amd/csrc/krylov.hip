@@ -415,20 +415,34 @@ __global__ __launch_bounds__(256) void k_prec_dot(long n, const T* __restrict__ 
   }
   block_store1(v, part);
 }
-// p = z + beta p ; first iteration p = z (p may hold uninitialised bits,
-// and 0 * NaN would not clear them)
+// The CG scalars stay on the device: rho (two slots, alternating by
+// iteration), p.q, and ||r||^2 per iteration in a window that the host reads
+// once per window (one synchronisation every W iterations instead of three
+// per iteration).  An exhausted direction (p.q == 0 or rho_old == 0) turns
+// the update into a no-op, so iterations run past convergence inside a
+// window leave x and r unchanged.
+constexpr int kCgRho = 0, kCgPq = 2, kCgHist = 4;
+
+// p = z + beta p, beta = rho_new / rho_old ; first iteration p = z (p may
+// hold uninitialised bits, and 0 * NaN would not clear them)
 template <class T>
-__global__ void k_xpby(long n, const T* __restrict__ z, double beta, int first, T* __restrict__ p) {
+__global__ void k_xpby(long n, const T* __restrict__ z, const double* __restrict__ cgs, int cur, int first,
+                       T* __restrict__ p) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = first ? z[i] : (T)fma(beta, (double)p[i], (double)z[i]);
+  if (i >= n) return;
+  const double rn = cgs[kCgRho + cur], ro = cgs[kCgRho + (cur ^ 1)];
+  const double beta = ro != 0.0 ? rn / ro : 0.0;
+  p[i] = first ? z[i] : (T)fma(beta, (double)p[i], (double)z[i]);
 }
-// x += alpha p ; r -= alpha q ; partial ||r||^2
+// x += alpha p ; r -= alpha q ; partial ||r||^2 ; alpha = rho / p.q
 template <class T>
-__global__ __launch_bounds__(256) void k_cg_update(long n, double alpha, const T* __restrict__ p,
-                                                   const T* __restrict__ q, T* __restrict__ x, T* __restrict__ r,
-                                                   double* __restrict__ part) {
+__global__ __launch_bounds__(256) void k_cg_update(long n, const double* __restrict__ cgs, int cur,
+                                                   const T* __restrict__ p, const T* __restrict__ q,
+                                                   T* __restrict__ x, T* __restrict__ r, double* __restrict__ part) {
+  const double pq = cgs[kCgPq];
+  const double al = pq != 0.0 ? cgs[kCgRho + cur] / pq : 0.0;
+  const T a = (T)al;
   double v = 0.0;
-  const T a = (T)alpha;
   for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
     x[i] = fma(a, p[i], x[i]);
     const T ri = fma(-a, q[i], r[i]);
@@ -444,6 +458,9 @@ static int cg_core(Ctx& c, const T* A, const Prec<T>& P, const T* b, T* x, doubl
   KrylovVecs<T>& v = vecs<T>(c);
   v.r.alloc(n); v.z.alloc(n); v.p.alloc(n); v.q.alloc(n);
   ensure_red(c, 1);
+  const int W = n < 2000000 ? 16 : 4;  // iterations per host check
+  if (c.kw.cgs.n < (size_t)(kCgHist + W)) c.kw.cgs.alloc(kCgHist + 16);
+  double* cgs = c.kw.cgs.p;
   const int nb = red_blocks(n);
   const double bn = norm2_t<T>(c, b, n);
   if (bn == 0.0) {
@@ -454,29 +471,42 @@ static int cg_core(Ctx& c, const T* A, const Prec<T>& P, const T* b, T* x, doubl
   spmv_t<T, T>(c, A, nullptr, x, v.q.p);
   hipLaunchKernelGGL(k_resid<T>, grid1(n), dim3(256), 0, c.stream, n, b, v.q.p, v.r.p);
   double rn = norm2_t<T>(c, v.r.p, n);
-  double rho = 1.0;
+  if (!std::isfinite(rn)) throw Error{MHX_ERR_NONFINITE, "CG: non-finite residual"};
   int it = 0;
+  std::vector<double> hist(W);
   while (it < maxit && rn / bn > tol) {
-    ScopedPhase ph(c.prof, "Krylov::cg_iteration", c.stream);
-    const double rho1 = rho;
-    if (P.kind == MHX_PREC_BLOCK_JACOBI) {
-      prec_apply<T>(c, P, v.r.p, v.z.p);
-      rho = reduce_to_host<T>(c, n, v.r.p, v.z.p, 0);
-    } else {
-      hipLaunchKernelGGL(k_prec_dot<T>, dim3(nb), dim3(256), 0, c.stream, n,
-                         P.kind == MHX_PREC_JACOBI ? P.d : (const T*)nullptr, v.r.p, v.z.p, c.kw.part.p);
-      rho = fetch(c, nb);
+    const int w = std::min(W, maxit - it);
+    {
+      ScopedPhase ph(c.prof, "Krylov::cg_window", c.stream);
+      for (int k = 0; k < w; ++k) {
+        const int cur = (it + k) & 1;
+        if (P.kind == MHX_PREC_BLOCK_JACOBI) {
+          prec_apply<T>(c, P, v.r.p, v.z.p);
+          hipLaunchKernelGGL(k_dot<T>, dim3(nb), dim3(256), 0, c.stream, n, v.r.p, v.z.p, c.kw.part.p, 0);
+        } else {
+          hipLaunchKernelGGL(k_prec_dot<T>, dim3(nb), dim3(256), 0, c.stream, n,
+                             P.kind == MHX_PREC_JACOBI ? P.d : (const T*)nullptr, v.r.p, v.z.p, c.kw.part.p);
+        }
+        hipLaunchKernelGGL(k_final, dim3(1), dim3(256), 0, c.stream, c.kw.part.p, nb, 1, 0, cgs + kCgRho + cur);
+        hipLaunchKernelGGL(k_xpby<T>, grid1(n), dim3(256), 0, c.stream, n, v.z.p, cgs, cur, it + k == 0 ? 1 : 0,
+                           v.p.p);
+        spmv_t<T, T>(c, A, nullptr, v.p.p, v.q.p);
+        hipLaunchKernelGGL(k_dot<T>, dim3(nb), dim3(256), 0, c.stream, n, v.p.p, v.q.p, c.kw.part.p, 0);
+        hipLaunchKernelGGL(k_final, dim3(1), dim3(256), 0, c.stream, c.kw.part.p, nb, 1, 0, cgs + kCgPq);
+        hipLaunchKernelGGL(k_cg_update<T>, dim3(nb), dim3(256), 0, c.stream, n, cgs, cur, v.p.p, v.q.p, x, v.r.p,
+                           c.kw.part.p);
+        hipLaunchKernelGGL(k_final, dim3(1), dim3(256), 0, c.stream, c.kw.part.p, nb, 1, 0, cgs + kCgHist + k);
+      }
+      MHX_HIP(hipGetLastError());
+      MHX_HIP(hipMemcpyAsync(hist.data(), cgs + kCgHist, w * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+      MHX_HIP(hipStreamSynchronize(c.stream));
     }
-    const double beta = it == 0 ? 0.0 : rho / rho1;
-    hipLaunchKernelGGL(k_xpby<T>, grid1(n), dim3(256), 0, c.stream, n, v.z.p, beta, it == 0 ? 1 : 0, v.p.p);
-    spmv_t<T, T>(c, A, nullptr, v.p.p, v.q.p);
-    const double pq = reduce_to_host<T>(c, n, v.p.p, v.q.p, 0);
-    const double alpha = rho / pq;
-    hipLaunchKernelGGL(k_cg_update<T>, dim3(nb), dim3(256), 0, c.stream, n, alpha, v.p.p, v.q.p, x, v.r.p,
-                       c.kw.part.p);
-    rn = std::sqrt(fetch(c, nb));
-    ++it;
-    if (!std::isfinite(rn)) throw Error{MHX_ERR_NONFINITE, "CG: non-finite residual"};
+    for (int k = 0; k < w; ++k) {
+      rn = std::sqrt(hist[k]);
+      ++it;
+      if (!std::isfinite(rn)) throw Error{MHX_ERR_NONFINITE, "CG: non-finite residual"};
+      if (rn / bn <= tol) break;
+    }
   }
   // true residual
   spmv_t<T, T>(c, A, nullptr, x, v.q.p);

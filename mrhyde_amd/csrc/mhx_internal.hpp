@@ -172,6 +172,7 @@ struct KrylovWork {
   DevBuf<float> jac32;
   DevBuf<double> x64;  // FP64 residual / correction of the refinement loop
   DevBuf<double> part, scal;
+  DevBuf<double> cgs;  // CG scalars on the device (rho, p.q, ||r||^2 window)
 };
 
 // ---------------------------------------------------------------------------
@@ -216,6 +217,8 @@ void lagrange_equispaced(int p, double xi, double* L, double* dL);
 bool thermal_supported(int dim, int p, int nq1);
 void assemble_thermal(Ctx& c, const double* u, double* res, double* jac, bool want_res, bool want_jac,
                       bool accumulate);
+// thermal Neumann boundary residual (bc.hip)
+void neumann_residual(Ctx& c, double* res);
 // navier stokes assembly (ns.hip)
 bool ns_supported(int dim, int p, int nq1);
 void assemble_ns(Ctx& c, const double* u, double* res, double* jac, bool want_res, bool want_jac, bool accumulate);

@@ -364,3 +364,70 @@ class SubgridBatch:
 
     def instance(self, x, i):
         return x[self.offsets[i]:self.offsets[i + 1]]
+
+
+class Server:
+    """Request loop over mhx_server_*: independent steady solves of mixed mesh
+    size, admitted by device-memory footprint, run concurrently (one stream
+    per active request), with idle contexts of the same mesh reused."""
+
+    def __init__(self, max_active: int = 8, max_cached: int = 32, hbm_budget: int = 0, solver: str = "cg",
+                 precond: str = "jacobi", tol: float = 1e-10, max_iters: int = 20000, newton_iters: int = 2,
+                 nl_tol: float = 1e-8, inner_tol: float = 0.0, restart: int = 50):
+        d = capi.ServerDesc()
+        d.hbm_budget, d.max_active, d.max_cached = int(hbm_budget), int(max_active), int(max_cached)
+        d.newton.max_iters, d.newton.nl_tol, d.newton.backtracking = int(newton_iters), float(nl_tol), 1
+        d.newton.linear = krylov_opts(solver=solver, tol=tol, max_iters=max_iters, restart=restart,
+                                      precond=precond, inner_tol=inner_tol)
+        h = C.c_void_p()
+        rc = lib.mhx_server_create(C.byref(d), C.byref(h))
+        if rc != 0:
+            raise MhxError(rc, "mhx_server_create")
+        self._h = h
+        self._problems = {}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib.mhx_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def submit(self, problem: Problem) -> int:
+        desc = problem.desc()
+        t = C.c_int64()
+        rc = lib.mhx_server_submit(self._h, C.byref(desc), C.byref(t))
+        if rc != 0:
+            raise MhxError(rc, "mhx_server_submit")
+        self._problems[t.value] = problem.n_dof
+        return t.value
+
+    def poll(self, ticket: int, wait: bool = True):
+        """(x, stats, latency_ms) once done, None while pending (wait=False)."""
+        x = np.empty(self._problems[ticket])
+        st = capi.SolveStats()
+        lat = C.c_double()
+        rc = lib.mhx_server_poll(self._h, int(ticket), int(wait), x.ctypes.data, C.byref(st), C.byref(lat))
+        if rc == 0:
+            return None
+        del self._problems[ticket]
+        if rc < 0:
+            raise MhxError(rc, "request failed: " + lib.mhx_server_last_error(self._h).decode())
+        return x, st.as_dict(), lat.value
+
+    def info(self) -> dict:
+        out = np.zeros(8, dtype=np.int64)
+        lib.mhx_server_info(self._h, out.ctypes.data)
+        keys = ["done", "active", "queued", "bytes_held", "reused", "created", "idle", "budget"]
+        return {k: int(v) for k, v in zip(keys, out)}
+
+    @staticmethod
+    def footprint(problem: Problem, opts: Optional[capi.KrylovOpts] = None):
+        desc = problem.desc()
+        n, z, b = C.c_int64(), C.c_int64(), C.c_int64()
+        rc = lib.mhx_footprint(C.byref(desc), C.byref(opts) if opts is not None else None, C.byref(n), C.byref(z),
+                               C.byref(b))
+        if rc != 0:
+            raise MhxError(rc, "mhx_footprint: " + lib.mhx_last_error(None).decode())
+        return n.value, z.value, b.value

@@ -489,6 +489,93 @@ static void assemble_thermal(const Problem& P, const double* u, bool want_jac, d
       if (P.fixed[g]) jv[find_slot(P, (int)g, (int)g)] = 1.0;
 }
 
+// thermal<EvalT>::boundaryResidual for "Neumann" sides (thermal.cpp:210-222),
+// run on the boundary groups after the volume groups (assemblyManager_jacres.hpp
+// :389-401): res(e, dof) += -g w_side phi_dof at the side cubature points,
+// scattered as res_glob -= res; g = q(x) . n does not depend on u, so the
+// Jacobian is unchanged.  Side cubature: the tensor Gauss rule of the volume on
+// the face axes; side weights w * |surface Jacobian| of the element's Q1 map
+// restricted to the face; n the outward unit normal.
+static void neumann_residual(const Problem& P, double* res) {
+  const mhx_problem_desc& d = P.desc;
+  if (!res || !d.neumann_sides || d.n_nq_modes <= 0) return;
+  const int dim = P.dim, p = P.p, nq1 = P.nq1, nb = P.nb;
+  std::vector<double> L(p + 1), dL(p + 1);
+  for (int s = 0; s < 2 * dim; ++s) {
+    if (!(d.neumann_sides & (1u << s))) continue;
+    const int a = s / 2;
+    const bool hi = s & 1;
+    const int ta[2] = {a == 0 ? 1 : 0, a == 2 ? 1 : 2};  // tangential axes (the first only in 2-D)
+    const int nt = dim - 1;
+    for (long e = 0; e < P.ne; ++e) {
+      long ec[3];
+      P.elem_coords(e, ec);
+      if (ec[a] != (hi ? P.nel[a] - 1 : 0)) continue;
+      std::vector<double> X(P.nv * 3);
+      for (int v = 0; v < P.nv; ++v) P.vertex_of_elem(e, v, &X[v * 3]);
+      const int nqs = nt == 1 ? nq1 : nq1 * nq1;
+      for (int qs = 0; qs < nqs; ++qs) {
+        double xi[3] = {0, 0, 0}, w = 1.0;
+        xi[a] = hi ? 1.0 : -1.0;
+        const int qq[2] = {qs % nq1, qs / nq1};
+        for (int t = 0; t < nt; ++t) {
+          xi[ta[t]] = P.qx1[qq[t]];
+          w *= P.qw1[qq[t]];
+        }
+        // Q1 map: x and the tangent vectors dx/dxi_t
+        double x[3] = {0, 0, 0}, tg[2][3] = {{0, 0, 0}, {0, 0, 0}};
+        for (int v = 0; v < P.nv; ++v) {
+          double N = 1.0, dN[3] = {1.0, 1.0, 1.0};
+          for (int b = 0; b < dim; ++b) {
+            const double sgn = ((v >> b) & 1) ? 1.0 : -1.0, f = 0.5 * (1.0 + sgn * xi[b]);
+            N *= f;
+            for (int c = 0; c < dim; ++c) dN[c] *= (c == b) ? 0.5 * sgn : f;
+          }
+          for (int c = 0; c < 3; ++c) {
+            x[c] += N * X[v * 3 + c];
+            for (int t = 0; t < nt; ++t) tg[t][c] += dN[ta[t]] * X[v * 3 + c];
+          }
+        }
+        double nrm[3] = {0, 0, 0}, area;
+        if (dim == 2) {
+          area = std::sqrt(tg[0][0] * tg[0][0] + tg[0][1] * tg[0][1]);
+          nrm[0] = tg[0][1] / area;
+          nrm[1] = -tg[0][0] / area;
+        } else {
+          nrm[0] = tg[0][1] * tg[1][2] - tg[0][2] * tg[1][1];
+          nrm[1] = tg[0][2] * tg[1][0] - tg[0][0] * tg[1][2];
+          nrm[2] = tg[0][0] * tg[1][1] - tg[0][1] * tg[1][0];
+          area = std::sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2]);
+          for (int c = 0; c < 3; ++c) nrm[c] /= area;
+        }
+        const double outward = hi ? 1.0 : -1.0;
+        if (nrm[a] * outward < 0) for (int c = 0; c < 3; ++c) nrm[c] = -nrm[c];
+        w *= area;
+        double g = 0.0;
+        for (int m = 0; m < d.n_nq_modes; ++m) {
+          double prod = 1.0;
+          for (int b = 0; b < dim; ++b) prod *= std::sin(d.nq_freq[m][b] * x[b] + d.nq_phase[m][b]);
+          for (int c = 0; c < dim; ++c) g += d.nq_amp[m][c] * prod * nrm[c];
+        }
+        // basis of every element DOF at the side point
+        std::vector<double> Lb(dim * (p + 1));
+        for (int b = 0; b < dim; ++b) {
+          lagrange1d(p, xi[b], L.data(), dL.data());
+          for (int k = 0; k <= p; ++k) Lb[b * (p + 1) + k] = L[k];
+        }
+        for (int j = 0; j < nb; ++j) {
+          const int jl[3] = {j % (p + 1), (j / (p + 1)) % (p + 1), dim == 3 ? j / ((p + 1) * (p + 1)) : 0};
+          double phi = 1.0;
+          for (int b = 0; b < dim; ++b) phi *= Lb[b * (p + 1) + jl[b]];
+          const int row = P.lids[e * nb + j];
+          if (P.fixed[row] || phi == 0.0) continue;
+          res[row] += -(-g * w * phi);
+        }
+      }
+    }
+  }
+}
+
 // navierstokes<EvalT>::computeTau (navierstokes.cpp:1053-1081), steady
 // (C3 = 0): tau = 1 / sqrt((C1 nu / h^2)^2 + (C2 |u| / h)^2), where |u|^2 is
 // replaced by its square root only when it exceeds 1e-12, as there.
@@ -623,17 +710,21 @@ static void assemble(const Problem& P, const double* u, bool want_jac, double* r
     else assemble_thermal<EvalT>(P, u, want_jac, res, jv);
   };
   struct D { using type = double; };
-  if (!want_jac) { run(D{}); return; }
-  switch (P.ad_size) {
-    case 2:  { struct T { using type = SFad<2>; }; run(T{}); break; }
-    case 4:  { struct T { using type = SFad<4>; }; run(T{}); break; }
-    case 8:  { struct T { using type = SFad<8>; }; run(T{}); break; }
-    case 16: { struct T { using type = SFad<16>; }; run(T{}); break; }
-    case 18: { struct T { using type = SFad<18>; }; run(T{}); break; }
-    case 24: { struct T { using type = SFad<24>; }; run(T{}); break; }
-    case 32: { struct T { using type = SFad<32>; }; run(T{}); break; }
-    default: { struct T { using type = SFad<64>; }; run(T{}); break; }
+  if (!want_jac) {
+    run(D{});
+  } else {
+    switch (P.ad_size) {
+      case 2:  { struct T { using type = SFad<2>; }; run(T{}); break; }
+      case 4:  { struct T { using type = SFad<4>; }; run(T{}); break; }
+      case 8:  { struct T { using type = SFad<8>; }; run(T{}); break; }
+      case 16: { struct T { using type = SFad<16>; }; run(T{}); break; }
+      case 18: { struct T { using type = SFad<18>; }; run(T{}); break; }
+      case 24: { struct T { using type = SFad<24>; }; run(T{}); break; }
+      case 32: { struct T { using type = SFad<32>; }; run(T{}); break; }
+      default: { struct T { using type = SFad<64>; }; run(T{}); break; }
+    }
   }
+  if (!ns) neumann_residual(P, res);  // boundary groups, after the volume groups
 }
 
 // ---------------------------------------------------------------------------

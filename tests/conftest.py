@@ -81,3 +81,19 @@ def ns_channel_problem(deck, nz=0, depth=0.0):
     return Problem(dim=dim, order=deck["order"], quadrature=deck["quadrature"], n=n, lo=lo, hi=hi,
                    physics=MHX_PHYS_NAVIER_STOKES, dbc_sides=deck["vel_dbc_sides"], rho=deck["rho"],
                    mu=deck["mu"], pspg=deck["pspg"], supg=deck["supg"], force=deck["force"], comp_sides=comp)
+
+
+def thermal_gold_problem(g):
+    """A thermal gold case of tests/golden/thermal_gold.json as a Problem, with
+    its true solution as (amplitude, frequencies) terms."""
+    from mrhyde_amd.abi import Problem
+
+    PI = np.pi
+    modes = [(a * PI * PI, [w * PI for w in f]) for a, f in g["source"]]
+    flux = [([a * PI for a in amp], [w * PI for w in fr], [ph * PI for ph in phs])
+            for amp, fr, phs in g.get("neumann_flux", [])]
+    p = Problem(dim=g["dim"], order=g["order"], quadrature=g["quadrature"], n=tuple(g["n"]), modes=modes,
+                workset_size=g["workset"], dbc_sides=g.get("dbc_sides", 63),
+                neumann_sides=g.get("neumann_sides", 0), neumann_flux=flux)
+    true = [(a, [w * PI for w in f]) for a, f in g["true"]]
+    return p, true, g

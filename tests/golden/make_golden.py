@@ -70,6 +70,11 @@ CASES = {
                             source=[[12.0, [2, 2, 2]]], true=[[1.0, [2, 2, 2]]], nl_iters=2, nl_tol=1e-7),
     "2D_verification_highorder": dict(dim=2, order=4, quadrature=8, n=[10, 10], workset=10,
                                       source=[[8.0, [2, 2]]], true=[[1.0, [2, 2]]], nl_iters=2, nl_tol=1e-11),
+    # Dirichlet on left/right, Neumann on bottom/top with the flux field
+    # q = (0, 2 pi sin(2 pi x) cos(2 pi y)): terms [amp, freq, phase], all in units of pi
+    "2D_mixed_bcs": dict(dim=2, order=1, quadrature=2, n=[40, 40], workset=100,
+                         source=[[8.0, [2, 2]]], true=[[1.0, [2, 2]]], nl_iters=4, nl_tol=1e-7,
+                         dbc_sides=3, neumann_sides=12, neumann_flux=[[[0.0, 2.0], [2.0, 2.0], [0.0, 0.5]]]),
 }
 
 

@@ -52,12 +52,9 @@ CASES = [k for k in GOLD if not k.startswith("_")]
 
 
 def gold_problem(case):
-    g = GOLD[case]
-    modes = [(a * PI * PI, [w * PI for w in f]) for a, f in g["source"]]
-    p = Problem(dim=g["dim"], order=g["order"], quadrature=g["quadrature"], n=tuple(g["n"]), modes=modes,
-                workset_size=g["workset"])
-    true = [(a, [w * PI for w in f]) for a, f in g["true"]]
-    return p, true, g
+    from conftest import thermal_gold_problem
+
+    return thermal_gold_problem(GOLD[case])
 
 
 @pytest.mark.parametrize("case", CASES)

@@ -42,6 +42,15 @@ def make_configs():
                                            modes=[(2.0, (PI, PI, PI)), (0.5, (2 * PI, 0, 3 * PI))],
                                            perturb=0.25, perturb_seed=9)
     cfg["3d_q2_quad2"] = Problem(dim=3, order=2, quadrature=2, n=(3, 2, 2), k0=lognormal(12, 0.5, 8))
+    # Neumann sides (thermal boundaryResidual)
+    flux2 = [((0.7, -1.3), (PI, 2 * PI), (0.3, 0.0)), ((0.0, 2.0), (2 * PI, 0.0), (0.0, PI / 2))]
+    cfg["2d_q2_neumann_perturbed"] = Problem(dim=2, order=2, quadrature=4, n=(6, 5), k0=lognormal(30, 0.5, 9),
+                                             diff_u2=0.5, dbc_sides=1, neumann_sides=2 | 4 | 8, neumann_flux=flux2,
+                                             perturb=0.2, perturb_seed=10)
+    flux3 = [((0.5, 1.0, -2.0), (PI, PI, 2 * PI), (0.0, 0.4, 0.0))]
+    cfg["3d_q2_neumann"] = Problem(dim=3, order=2, quadrature=4, n=(4, 3, 3), k0=lognormal(36, 0.5, 11),
+                                   diff_u2=1.0, dbc_sides=1 | 2, neumann_sides=4 | 16 | 32, neumann_flux=flux3,
+                                   perturb=0.2, perturb_seed=12)
     # navier stokes, Q1-Q1
     cfg["2d_ns_channel_pspg"] = Problem(dim=2, order=1, quadrature=2, n=(10, 4), hi=(5.0, 1.0),
                                         physics=MHX_PHYS_NAVIER_STOKES, dbc_sides=4 | 8, pspg=True,
@@ -227,10 +236,10 @@ GOLD = json.load(open(os.path.join(GOLDEN, "thermal_gold.json")))
 def test_gold_l2_on_gpu(case):
     from mrhyde_amd import Mhx
 
-    gd = GOLD[case]
-    modes = [(a * PI * PI, [w * PI for w in f]) for a, f in gd["source"]]
-    true = [(a, [w * PI for w in f]) for a, f in gd["true"]]
-    g = Mhx(Problem(dim=gd["dim"], order=gd["order"], quadrature=gd["quadrature"], n=tuple(gd["n"]), modes=modes))
+    from conftest import thermal_gold_problem
+
+    p, true, gd = thermal_gold_problem(GOLD[case])
+    g = Mhx(p)
     u = g.setDirichlet(np.zeros(g.n_dof))
     u, st = g.nonlinearSolver(u, max_iters=gd["nl_iters"], nl_tol=gd["nl_tol"], solver="cg", tol=1e-12,
                               linear_max_iters=20000)
