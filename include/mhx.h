@@ -231,7 +231,9 @@ int  mhx_l2_error(mhx_ctx* ctx, const double* u, int32_t n_modes, const double* 
  *   "assembly.colouring"  0 = element colouring (default: one launch per
  *                         element colour), 1 = block colouring (one launch
  *                         per block colour, element colours as in-workgroup
- *                         sub-phases).
+ *                         sub-phases), 2 = x-segments (a workgroup merges a
+ *                         run of x-adjacent elements in LDS and writes each
+ *                         row's share as contiguous runs; thermal only).
  * Returns MHX_ERR_ARG for an unknown name or value. */
 int  mhx_set_option(mhx_ctx* ctx, const char* name, int64_t value);
 int  mhx_get_option(const mhx_ctx* ctx, const char* name, int64_t* value);

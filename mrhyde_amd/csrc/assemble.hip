@@ -304,10 +304,51 @@ __device__ __forceinline__ void test_apply(const Tab1& t, FQ&& fq, double* R) {
   }
 }
 
-template <int DIM, int P, int NQ1, bool JAC, bool BLOCKED>
+// LDS image of a segment's Jacobian rows (x-segment scatter).  The segment's
+// lattice nodes form x-planes 0..P*S; a plane holds RP rows (the pencil's
+// y/z nodes), and each row's share of columns from this segment is CP (y,z)
+// column blocks of W x-columns: W = 2P+1 on vertex planes (every P-th plane,
+// shared by two elements), P+1 on interior planes.  The order within a row,
+// (z', y', x'), is the CRS order, so each (row, z') block of the image is
+// one contiguous run of the CRS row when it is written out.
+template <int DIM, int P, int S>
+struct SegCfg {
+  static constexpr int P1 = P + 1;
+  static constexpr int RP = ipow(P1, DIM - 1);
+  static constexpr int CP = RP;
+  static constexpr int WV = 2 * P + 1, WI = P + 1;
+  static constexpr int SV = RP * CP * WV, SI = RP * CP * WI;
+  static constexpr int GS = SV + (P - 1) * SI;  // a vertex plane and the interior planes after it
+  static constexpr int NPL = P * S + 1;
+  static constexpr int kSize = S * GS + SV;
+  static constexpr int kRows = NPL * RP;
+  __device__ static constexpr int plane_off(int X) {
+    return (X / P) * GS + (X % P == 0 ? 0 : SV + (X % P - 1) * SI);
+  }
+};
+
+// lattice range of the columns of a row along one axis (node c of n)
+__device__ __forceinline__ void row_range(int c, int p, int n, int& lo, int& len) {
+  if (c % p == 0) {
+    lo = c - p < 0 ? 0 : c - p;
+    const int hi = c + p > n - 1 ? n - 1 : c + p;
+    len = hi - lo + 1;
+  } else {
+    lo = c - c % p;
+    len = p + 1;
+  }
+}
+
+template <int DIM, int P, int NQ1, bool JAC, int MODE>
 __global__ __launch_bounds__(256) void k_thermal(AsmArgs a) {
   using C = Cfg<DIM, P, NQ1>;
   constexpr int NB = C::NB, NQ = C::NQ, NV = C::NV, NC = C::NC, G = C::G, GPB = C::GPB, P1 = C::P1;
+  constexpr bool BLOCKED = MODE == kBlockColouring;
+  constexpr bool SEG = MODE == kSegmentMerge;
+  using SG = SegCfg<DIM, P, GPB>;
+  __shared__ double s_seg[SEG && JAC ? SG::kSize : 1];
+  __shared__ double s_sres[SEG ? SG::kRows : 1];
+  __shared__ int s_srp[SEG ? SG::kRows : 1];
   // rows of Dhat that can be nonzero: the flux rows, plus the source row when
   // the source depends on the solution
   constexpr int NR = ThermalPhysics::kSrcDependsOnU ? NC : DIM;
@@ -326,13 +367,20 @@ __global__ __launch_bounds__(256) void k_thermal(AsmArgs a) {
   for (int sub = 0; sub < kMaxSub; ++sub) {
   if (sub) __syncthreads();  // previous sub-phase's LDS reads and global stores are complete
   int ec[3];
-  const bool active = element_of<DIM>(a, g, sub, ec);
+  int sx = 0, s_act = 0;
+  const bool active = SEG ? element_of_seg<DIM>(a, g, ec, sx, s_act) : element_of<DIM>(a, g, sub, ec);
   const long e = ec[0] + (long)a.m.nel[0] * (ec[1] + (long)a.m.nel[1] * ec[2]);
-  // first-touch flags per axis: is this element the first scheduled one among
-  // the (at most two) elements sharing its low / high vertex plane
+  // first-touch flags per axis: is this element (segment, along x in segment
+  // scheduling) the first scheduled one among the (at most two) sharing its
+  // low / high vertex plane
   bool first_lo[3] = {true, true, true}, first_hi[3] = {true, true, true};
 #pragma unroll
   for (int d = 0; d < DIM; ++d) {
+    if (SEG && d == 0) {
+      first_lo[0] = sx == 0 || (sx & 1) < ((sx - 1) & 1);
+      first_hi[0] = sx == a.nseg - 1 || (sx & 1) < ((sx + 1) & 1);
+      continue;
+    }
     const int k = axis_key(a, d, ec[d]);
     first_lo[d] = ec[d] == 0 || k < axis_key(a, d, ec[d] - 1);
     first_hi[d] = ec[d] == a.m.nel[d] - 1 || k < axis_key(a, d, ec[d] + 1);
@@ -593,6 +641,154 @@ __global__ __launch_bounds__(256) void k_thermal(AsmArgs a) {
   }
   __syncthreads();  // all reads of s_u done before it is reused
 
+  if constexpr (SEG) {
+    // 4'. segment scatter: the segment's element columns are summed into an
+    // LDS image of its rows (groups of even, then odd index, so the two
+    // elements sharing a vertex plane never write at once), then the
+    // workgroup writes the image out row run by row run.  An entry is shared
+    // with another work unit only on the segment's end planes (x) or the
+    // pencil's faces (y, z); the first scheduled unit stores it, later ones
+    // add into it, as in element colouring.
+    const int X0g = P * sx * GPB;  // global x-node of plane 0
+    const int Xend = P * s_act;    // last plane holding this segment's entries
+    const int yb = P * ec[1], zb = P * ec[2];
+    if constexpr (JAC)
+      for (int k = threadIdx.x; k < SG::kSize; k += 256) s_seg[k] = 0.0;
+    for (int k = threadIdx.x; k < SG::kRows; k += 256) {
+      s_sres[k] = 0.0;
+      const int X = k / SG::RP, r = k % SG::RP;
+      const int Xg = X0g + X;
+      if (X <= Xend && Xg < a.m.nn[0]) {
+        const long row = Xg + (long)a.m.nn[0] * ((yb + r % P1) + (long)a.m.nn[1] * (zb + (DIM == 3 ? r / P1 : 0)));
+        s_srp[k] = a.fixed[row] ? -1 - a.rowptr[row] : a.rowptr[row];
+      }
+    }
+    __syncthreads();
+    for (int ph = 0; ph < 2; ++ph) {
+      if (active && (g & 1) == ph) {
+        if constexpr (JAC) {
+          if (lane < NB) {
+            const int jl[3] = {lane % P1, (lane / P1) % P1, DIM == 3 ? lane / (P1 * P1) : 0};
+#pragma unroll
+            for (int i = 0; i < NB; ++i) {
+              const int ix = i % P1, iy = (i / P1) % P1, iz = DIM == 3 ? i / (P1 * P1) : 0;
+              const bool vtx = ix == 0 || ix == P;
+              const int W = vtx ? SG::WV : SG::WI;
+              const int xo = vtx ? jl[0] - ix + P : jl[0];
+              const int idx = SG::plane_off(P * g + ix) + ((iy + P1 * iz) * SG::CP + (jl[2] * P1 + jl[1])) * W + xo;
+              s_seg[idx] += R[i];
+            }
+          } else if (lane == NB && a.res) {
+#pragma unroll
+            for (int i = 0; i < NB; ++i) {
+              const int ix = i % P1, iy = (i / P1) % P1, iz = DIM == 3 ? i / (P1 * P1) : 0;
+              s_sres[(P * g + ix) * SG::RP + iy + P1 * iz] += R[i];
+            }
+          }
+        } else {
+          if (lane < NB) {
+            const int ix = lane % P1, iy = (lane / P1) % P1, iz = DIM == 3 ? lane / (P1 * P1) : 0;
+            s_sres[(P * g + ix) * SG::RP + iy + P1 * iz] += R[0];
+          }
+        }
+      }
+      __syncthreads();
+    }
+    // residual rows
+    if (a.res) {
+      for (int k = threadIdx.x; k < SG::kRows; k += 256) {
+        const int X = k / SG::RP, r = k % SG::RP;
+        const int Xg = X0g + X;
+        if (X > Xend || Xg >= a.m.nn[0]) continue;
+        const int iy = r % P1, iz = DIM == 3 ? r / P1 : 0;
+        bool sh = true;
+        if (X == 0) sh = sh && first_lo[0];
+        if (X == Xend) sh = sh && first_hi[0];
+        if (iy == 0) sh = sh && first_lo[1];
+        if (iy == P) sh = sh && first_hi[1];
+        if (DIM == 3 && iz == 0) sh = sh && first_lo[2];
+        if (DIM == 3 && iz == P) sh = sh && first_hi[2];
+        const long row = Xg + (long)a.m.nn[0] * ((yb + iy) + (long)a.m.nn[1] * (zb + iz));
+        const bool fx = s_srp[k] < 0;
+        const double v = -s_sres[k];
+        if (a.first_touch && sh) a.res[row] = fx ? 0.0 : v;
+        else if (!fx) a.res[row] += v;
+      }
+    }
+    // Jacobian rows, CH entries per thread per pass: slots and flags, then
+    // the loads of entries that earlier units already hold, then the stores
+    if constexpr (JAC) {
+      if (a.jac) {
+        constexpr int CH = 8;
+        for (int base = threadIdx.x; base < SG::kSize; base += 256 * CH) {
+          int sl[CH], mode[CH];
+          double val[CH], old[CH];
+#pragma unroll
+          for (int c = 0; c < CH; ++c) {
+            mode[c] = 0;
+            sl[c] = 0;
+            val[c] = 0.0;
+            const int eix = base + c * 256;
+            if (eix >= SG::kSize) continue;
+            // decode (plane, row, column block, x offset); every divisor is a
+            // compile-time constant (vertex and interior planes separately)
+            const int grp = eix / SG::GS, rem = eix % SG::GS;
+            int X, r, cb, xo;
+            if (rem < SG::SV) {
+              X = P * grp;
+              r = rem / (SG::CP * SG::WV);
+              cb = (rem / SG::WV) % SG::CP;
+              xo = rem % SG::WV;
+            } else {
+              const int r2 = rem - SG::SV;
+              X = P * grp + 1 + r2 / SG::SI;
+              const int rr = r2 % SG::SI;
+              r = rr / (SG::CP * SG::WI);
+              cb = (rr / SG::WI) % SG::CP;
+              xo = rr % SG::WI;
+            }
+            const int Xg = X0g + X;
+            if (X > Xend || Xg >= a.m.nn[0]) continue;
+            const int xcol = (X % P == 0) ? Xg - P + xo : Xg - X % P + xo;
+            if (xcol < 0 || xcol >= a.m.nn[0]) continue;
+            if ((X == 0 && xcol < Xg) || (X == Xend && xcol > Xg)) continue;  // a neighbouring segment's entries
+            const int iy = r % P1, iz = DIM == 3 ? r / P1 : 0;
+            const int jy = cb % P1, jz = DIM == 3 ? cb / P1 : 0;
+            int lox, lx, loy, ly, loz = 0, lz = 1;
+            row_range(Xg, P, a.m.nn[0], lox, lx);
+            row_range(yb + iy, P, a.m.nn[1], loy, ly);
+            if (DIM == 3) row_range(zb + iz, P, a.m.nn[2], loz, lz);
+            int rp = s_srp[X * SG::RP + r];
+            const bool fx = rp < 0;
+            if (fx) rp = -1 - rp;
+            sl[c] = rp + (((zb + jz - loz) * ly + (yb + jy - loy)) * lx + (xcol - lox));
+            bool sh = true;
+            if (X == 0 && xcol == Xg) sh = sh && first_lo[0];
+            if (X == Xend && xcol == Xg) sh = sh && first_hi[0];
+            if (iy == jy && iy == 0) sh = sh && first_lo[1];
+            if (iy == jy && iy == P) sh = sh && first_hi[1];
+            if (DIM == 3 && iz == jz && iz == 0) sh = sh && first_lo[2];
+            if (DIM == 3 && iz == jz && iz == P) sh = sh && first_hi[2];
+            const bool first = a.first_touch && sh;
+            if (fx) {
+              if (first) {
+                mode[c] = 1;
+                val[c] = (xcol == Xg && jy == iy && jz == iz) ? 1.0 : 0.0;
+              }
+            } else {
+              mode[c] = first ? 1 : 2;
+              val[c] = s_seg[eix];
+            }
+          }
+#pragma unroll
+          for (int c = 0; c < CH; ++c) old[c] = mode[c] == 2 ? a.jac[sl[c]] : 0.0;
+#pragma unroll
+          for (int c = 0; c < CH; ++c)
+            if (mode[c]) a.jac[sl[c]] = val[c] + old[c];
+        }
+      }
+    }
+  } else {
   // 4. scatter -------------------------------------------------------------------
   // Lane i fetches row i's CRS offset once (negated minus one for a
   // strong-Dirichlet row); the value lane hands the residual to the row lanes
@@ -669,15 +865,18 @@ __global__ __launch_bounds__(256) void k_thermal(AsmArgs a) {
           const int i = i0 + k;
           if (i >= NB) break;
           const bool first = (first_mask >> k) & 1u, fx = (fixed_mask >> k) & 1u;
+          // streaming stores: a later colour re-reads an entry long after
+          // this launch, so keeping the line in L2 buys nothing
           if (fx) {
-            if (first) a.jac[sl[k]] = (i == lane) ? 1.0 : 0.0;
+            if (first) __builtin_nontemporal_store((i == lane) ? 1.0 : 0.0, &a.jac[sl[k]]);
           } else {
-            a.jac[sl[k]] = old[k] + R[i];
+            __builtin_nontemporal_store(old[k] + R[i], &a.jac[sl[k]]);
           }
         }
       }
     }
   }
+  }  // element / block colouring scatter
   }  // sub-phase loop
 }
 
@@ -717,9 +916,11 @@ template <int DIM, int P, int NQ1>
 static void launch_colors(Ctx& c, AsmArgs& a, bool jac) {
   using C = Cfg<DIM, P, NQ1>;
   const int ncol = 1 << DIM;
+  const int mode = c.scatter_mode;
   a.gpb = C::GPB;
-  a.blocked = c.scatter_mode == kBlockColouring;
-  a.nsub = a.blocked ? ncol : 1;
+  a.blocked = mode;
+  a.nsub = mode == kBlockColouring ? ncol : 1;
+  a.nseg = (c.mesh.nel[0] + C::GPB - 1) / C::GPB;
   block_shape(DIM, C::GPB, a.s);
   for (int d = 0; d < 3; ++d) a.bw[d] = 2 * a.s[d];
   for (int col = 0; col < ncol; ++col) {
@@ -727,8 +928,11 @@ static void launch_colors(Ctx& c, AsmArgs& a, bool jac) {
     long cnt = 1;
     for (int d = 0; d < 3; ++d) {
       const int cd = (col >> d) & 1;
-      // units along the axis: elements (element colouring) or blocks (block colouring)
-      const int units = d < DIM ? (a.blocked ? (c.mesh.nel[d] + a.bw[d] - 1) / a.bw[d] : c.mesh.nel[d]) : 1;
+      // units along the axis: elements (element colouring), blocks (block
+      // colouring) or, along x, segments (segment scheduling)
+      int units = d < DIM ? c.mesh.nel[d] : 1;
+      if (d < DIM && mode == kBlockColouring) units = (c.mesh.nel[d] + a.bw[d] - 1) / a.bw[d];
+      if (d == 0 && mode == kSegmentMerge) units = a.nseg;
       int n = d < DIM ? (units - cd + 1) / 2 : 1;
       if (d >= DIM && cd) n = 0;
       a.cn[d] = n;
@@ -736,18 +940,18 @@ static void launch_colors(Ctx& c, AsmArgs& a, bool jac) {
     }
     a.ncolor = cnt;
     if (cnt == 0) continue;
-    const long blocks = a.blocked ? cnt : (cnt + C::GPB - 1) / C::GPB;
+    const long blocks = mode == kElementColouring ? (cnt + C::GPB - 1) / C::GPB : cnt;
     ScopedPhase ph(c.prof, jac ? "k_thermal<jacres>" : "k_thermal<res>", c.stream);
-    if (a.blocked) {
-      if (jac)
-        hipLaunchKernelGGL((k_thermal<DIM, P, NQ1, true, true>), dim3((unsigned)blocks), dim3(C::BLOCK), 0, c.stream, a);
-      else
-        hipLaunchKernelGGL((k_thermal<DIM, P, NQ1, false, true>), dim3((unsigned)blocks), dim3(C::BLOCK), 0, c.stream, a);
+    const dim3 grid((unsigned)blocks), block(C::BLOCK);
+    if (mode == kSegmentMerge) {
+      if (jac) hipLaunchKernelGGL((k_thermal<DIM, P, NQ1, true, kSegmentMerge>), grid, block, 0, c.stream, a);
+      else hipLaunchKernelGGL((k_thermal<DIM, P, NQ1, false, kSegmentMerge>), grid, block, 0, c.stream, a);
+    } else if (mode == kBlockColouring) {
+      if (jac) hipLaunchKernelGGL((k_thermal<DIM, P, NQ1, true, kBlockColouring>), grid, block, 0, c.stream, a);
+      else hipLaunchKernelGGL((k_thermal<DIM, P, NQ1, false, kBlockColouring>), grid, block, 0, c.stream, a);
     } else {
-      if (jac)
-        hipLaunchKernelGGL((k_thermal<DIM, P, NQ1, true, false>), dim3((unsigned)blocks), dim3(C::BLOCK), 0, c.stream, a);
-      else
-        hipLaunchKernelGGL((k_thermal<DIM, P, NQ1, false, false>), dim3((unsigned)blocks), dim3(C::BLOCK), 0, c.stream, a);
+      if (jac) hipLaunchKernelGGL((k_thermal<DIM, P, NQ1, true, kElementColouring>), grid, block, 0, c.stream, a);
+      else hipLaunchKernelGGL((k_thermal<DIM, P, NQ1, false, kElementColouring>), grid, block, 0, c.stream, a);
     }
     MHX_HIP(hipGetLastError());
   }

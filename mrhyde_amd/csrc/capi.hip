@@ -699,8 +699,8 @@ int mhx_set_option(mhx_ctx* ctx, const char* name, int64_t value) {
   if (!ctx || !name) return MHX_ERR_ARG;
   const std::string k(name);
   if (k == "assembly.colouring") {
-    if (value != kElementColouring && value != kBlockColouring) {
-      ctx->err = "assembly.colouring must be 0 or 1";
+    if (value != kElementColouring && value != kBlockColouring && value != kSegmentMerge) {
+      ctx->err = "assembly.colouring must be 0, 1 or 2";
       return MHX_ERR_ARG;
     }
     ctx->scatter_mode = (int)value;

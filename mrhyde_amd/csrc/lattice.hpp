@@ -64,11 +64,31 @@ __device__ __forceinline__ bool element_of(const AsmArgs& a, int g, int sub, int
   return ok && g < a.gpb;
 }
 
+// Segment scheduling (a.blocked == 2): a workgroup owns a segment of a.gpb
+// x-consecutive elements at one (ey, ez); group g takes the segment's g-th
+// element.  One launch per colour of (segment parity, ey parity, ez parity):
+// same-colour segments are a segment apart in x or an element apart in y/z,
+// so they share no node.  Returns whether group g has an element (the last
+// segment of a row may be short) and the segment index and length.
+template <int DIM>
+__device__ __forceinline__ bool element_of_seg(const AsmArgs& a, int g, int* ec, int& sx, int& s_act) {
+  int t = blockIdx.x;
+  sx = 2 * (t % a.cn[0]) + (a.color & 1);
+  t /= a.cn[0];
+  ec[1] = 2 * (t % a.cn[1]) + ((a.color >> 1) & 1);
+  t /= a.cn[1];
+  ec[2] = DIM == 3 ? 2 * t + ((a.color >> 2) & 1) : 0;
+  ec[0] = sx * a.gpb + g;
+  const int rest = a.m.nel[0] - sx * a.gpb;
+  s_act = rest < a.gpb ? rest : a.gpb;
+  return g < s_act;
+}
+
 // Scheduling key of element index e along one axis: (block colour bit,
 // element colour bit).  Launch order is block colour major, sub-phase minor,
 // so along each axis the smaller key runs first.
 __device__ __forceinline__ int axis_key(const AsmArgs& a, int d, int e) {
-  return (a.blocked ? ((e / a.bw[d]) & 1) * 2 : 0) + (e & 1);
+  return (a.blocked == 1 ? ((e / a.bw[d]) & 1) * 2 : 0) + (e & 1);
 }
 
 }  // namespace mhx

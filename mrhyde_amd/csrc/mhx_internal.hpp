@@ -81,7 +81,8 @@ struct AsmArgs {
   int color;              // colour bits (cx | cy<<1 | cz<<2): element or block colour
   long ncolor;            // elements (element colouring) / blocks (block colouring) of this colour
   int cn[3];              // per-axis count inside this colour
-  int blocked;            // 0: element colouring, 1: block colouring
+  int blocked;            // 0: element colouring, 1: block colouring, 2: x-segments
+  int nseg;               // segments per x-row (segment scheduling)
   int nsub;               // sub-phases per workgroup (1, or 2^dim element colours)
   int gpb;                // element groups per workgroup
   int s[3];               // block colouring: groups per axis (prod = gpb)
@@ -89,7 +90,7 @@ struct AsmArgs {
   int first_touch;        // overwrite on first touch instead of accumulate
 };
 
-enum ScatterMode { kElementColouring = 0, kBlockColouring = 1 };
+enum ScatterMode { kElementColouring = 0, kBlockColouring = 1, kSegmentMerge = 2 };
 
 // ---------------------------------------------------------------------------
 // device buffers

@@ -386,10 +386,11 @@ __global__ __launch_bounds__(256) void k_ns(AsmArgs a, NsCoef nc) {
           const int i = i0 + k;
           if (i >= NE) break;
           const bool first = (first_mask >> k) & 1u, fx = (fixed_mask >> k) & 1u;
+          // streaming stores (a later colour re-reads the entry long after this launch)
           if (fx) {
-            if (first) a.jac[sl[k]] = (i == lane) ? 1.0 : 0.0;
+            if (first) __builtin_nontemporal_store((i == lane) ? 1.0 : 0.0, &a.jac[sl[k]]);
           } else {
-            a.jac[sl[k]] = old[k] + R[i];
+            __builtin_nontemporal_store(old[k] + R[i], &a.jac[sl[k]]);
           }
         }
       }

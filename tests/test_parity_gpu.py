@@ -94,7 +94,7 @@ def test_pattern_and_dirichlet_bit_exact(pairs, name):
     np.testing.assert_allclose(g.dof_coords(), o.dof_coords(), rtol=0, atol=1e-15)
 
 
-@pytest.mark.parametrize("colouring", [1, 0])
+@pytest.mark.parametrize("colouring", [2, 1, 0])
 @pytest.mark.parametrize("name", list(CONFIGS))
 def test_assembly_parity(pairs, name, colouring):
     g, o = pairs[name]
@@ -114,7 +114,7 @@ def test_assembly_parity(pairs, name, colouring):
     g.set_option("assembly.colouring", 1)
 
 
-@pytest.mark.parametrize("colouring", [1, 0])
+@pytest.mark.parametrize("colouring", [2, 1, 0])
 def test_accumulate_and_device_pointers(pairs, colouring):
     from mrhyde_amd.abi import MHX_BUF_JAC, MHX_BUF_RES, MHX_BUF_U
 
