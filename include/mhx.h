@@ -234,6 +234,9 @@ int  mhx_l2_error(mhx_ctx* ctx, const double* u, int32_t n_modes, const double* 
  *                         sub-phases), 2 = x-segments (a workgroup merges a
  *                         run of x-adjacent elements in LDS and writes each
  *                         row's share as contiguous runs; thermal only).
+ *   "krylov.spmv_lanes"   lanes that share one CRS row in the sparse
+ *                         matrix-vector product: 4, 8, 16, 32 or 64 (default
+ *                         chosen from the mean row length at context build).
  * Returns MHX_ERR_ARG for an unknown name or value. */
 int  mhx_set_option(mhx_ctx* ctx, const char* name, int64_t value);
 int  mhx_get_option(const mhx_ctx* ctx, const char* name, int64_t* value);

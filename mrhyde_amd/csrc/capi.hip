@@ -706,6 +706,14 @@ int mhx_set_option(mhx_ctx* ctx, const char* name, int64_t value) {
     ctx->scatter_mode = (int)value;
     return MHX_OK;
   }
+  if (k == "krylov.spmv_lanes") {
+    if (value != 4 && value != 8 && value != 16 && value != 32 && value != 64) {
+      ctx->err = "krylov.spmv_lanes must be 4, 8, 16, 32 or 64";
+      return MHX_ERR_ARG;
+    }
+    ctx->spmv_lanes = (int)value;
+    return MHX_OK;
+  }
   ctx->err = "unknown option " + k;
   return MHX_ERR_ARG;
 }
@@ -715,6 +723,10 @@ int mhx_get_option(const mhx_ctx* ctx, const char* name, int64_t* value) {
   const std::string k(name);
   if (k == "assembly.colouring") {
     *value = ctx->scatter_mode;
+    return MHX_OK;
+  }
+  if (k == "krylov.spmv_lanes") {
+    *value = ctx->spmv_lanes;
     return MHX_OK;
   }
   return MHX_ERR_ARG;

@@ -38,13 +38,23 @@ __global__ __launch_bounds__(256) void k_spmv(long n, const int32_t* __restrict_
   if (row >= n) return;
   const int b = rp[row], e = rp[row + 1];
   TX acc = 0;
-  if (s) {
-    for (int k = b + l; k < e; k += LANES) {
-      const int c = ci[k];
-      acc += (TX)A[k] * (s[c] * x[c]);
+  // four entries per pass: the value and index loads of all four are issued
+  // before the dependent gathers of x, so a lane keeps several misses in flight
+  int k = b + l;
+  for (; k + 3 * LANES < e; k += 4 * LANES) {
+    TA av[4];
+    int cv[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      av[u] = A[k + u * LANES];
+      cv[u] = ci[k + u * LANES];
     }
-  } else {
-    for (int k = b + l; k < e; k += LANES) acc += (TX)A[k] * x[ci[k]];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) acc += (TX)av[u] * (s ? s[cv[u]] * x[cv[u]] : x[cv[u]]);
+  }
+  for (; k < e; k += LANES) {
+    const int c = ci[k];
+    acc += (TX)A[k] * (s ? s[c] * x[c] : x[c]);
   }
 #pragma unroll
   for (int off = LANES / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, LANES);

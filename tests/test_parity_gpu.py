@@ -163,6 +163,28 @@ def test_linear_solve_parity(pairs, name, solver):
     np.testing.assert_allclose(y, spmv_np(rp, ci, jo, x), rtol=1e-13, atol=1e-14 * np.abs(y).max())
 
 
+@pytest.mark.parametrize("name", ["3d_q2_c2shape", "3d_ns_lid_cavity", "2d_q1_poisson_c1shape"])
+def test_spmv_lane_widths(pairs, name):
+    """The CRS product at every lane count per row (rows of 9 to 125 entries:
+    lanes with several four-entry passes, one pass, and only the tail loop)."""
+    g, o = pairs[name]
+    u = random_state(o, 11)
+    _, jo = o.assembleJacRes(u)
+    rp, ci = o.crs_pattern()
+    x = np.random.default_rng(12).standard_normal(o.n_dof)
+    ref = spmv_np(rp, ci, jo, x)
+    default = g.get_option("krylov.spmv_lanes")
+    for lanes in (4, 8, 16, 32, 64):
+        g.set_option("krylov.spmv_lanes", lanes)
+        y = g.apply(jo, x)
+        np.testing.assert_allclose(y, ref, rtol=1e-13, atol=1e-14 * np.abs(ref).max(), err_msg=f"lanes={lanes}")
+    g.set_option("krylov.spmv_lanes", default)
+    from mrhyde_amd import MhxError
+
+    with pytest.raises(MhxError):
+        g.set_option("krylov.spmv_lanes", 12)
+
+
 @pytest.mark.parametrize("name,solver", [("2d_q1_poisson_c1shape", "cg"), ("3d_q1_lognormal", "cg"),
                                          ("3d_q2_c2shape", "gmres"), ("2d_q4_quad8", "gmres")])
 def test_mixed_precision_solve(pairs, name, solver):
