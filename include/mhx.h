@@ -128,6 +128,9 @@ typedef struct mhx_ctx mhx_ctx;
 #define MHX_PREC_NONE     0
 #define MHX_PREC_JACOBI   1   /* Ifpack2 RELAXATION "Jacobi"                   */
 #define MHX_PREC_BLOCK_JACOBI 2  /* nodal block Jacobi (multi-field)           */
+#define MHX_PREC_AMG      3   /* smoothed-aggregation AMG, Chebyshev(2) V-cycle:
+                                 the reference's default MueLu preconditioner,
+                                 linearAlgebraInterface_solvers.hpp:201-253   */
 #define MHX_PREC_FP32     0x100  /* inner Krylov + preconditioner in FP32, FP64 refinement */
 
 typedef struct mhx_krylov_opts {
@@ -209,6 +212,24 @@ int  mhx_spmv(mhx_ctx* ctx, const double* jac_vals, const double* x, double* y, 
  * and SolverManager::PCG (src/managers/solver/solverManager_solvers.hpp:812-905). */
 int  mhx_linear_solve(mhx_ctx* ctx, const double* jac_vals, const double* r, double* x,
                       const mhx_krylov_opts* opts, mhx_solve_stats* stats);
+
+/* Build the smoothed-aggregation hierarchy of J (jac_vals == NULL: the
+ * context's Jacobian; MHX_DEVICE_PTRS in flags: device pointer) and keep it
+ * in the context; a solve with MHX_PREC_AMG rebuilds it from its own J.
+ * Returns the number of levels.  Replaces
+ * LinearAlgebraInterface::buildAMGPreconditioner
+ * (src/interfaces/linear_algebra/linearAlgebraInterface_solvers.hpp:201-253). */
+int  mhx_amg_setup(mhx_ctx* ctx, const double* jac_vals, int32_t flags);
+
+/* One operator of the hierarchy held by the context, copied to host arrays
+ * (any of rp/ci/vals may be NULL to query sizes only): which = 0 the level's
+ * operator A_l (CRS), 1 its prolongator P_l, 2 its restriction R_l = P_l^T,
+ * 3 the dense inverse of the coarsest operator (row-major in vals).
+ * info[0..3] = rows, cols, nnz, number of levels; *lambda (if not NULL) =
+ * the level's power-iteration estimate of rho(D^-1 A_l).
+ * Introspection of MueLu::Hierarchy levels; no single reference call. */
+int  mhx_amg_level(mhx_ctx* ctx, int32_t level, int32_t which, int64_t info[4], double* lambda, int32_t* rp,
+                   int32_t* ci, double* vals);
 
 /* Newton on u in place (u must already carry the Dirichlet values, see
  * mhx_set_dirichlet).  Replaces SolverManager::nonlinearSolver

@@ -28,7 +28,7 @@ MHX_SIDE_ALL = 63
 
 MHX_RES, MHX_JAC, MHX_ACCUMULATE, MHX_DEVICE_PTRS = 1, 2, 4, 8
 MHX_KRYLOV_CG, MHX_KRYLOV_GMRES = 1, 2
-MHX_PREC_NONE, MHX_PREC_JACOBI, MHX_PREC_BLOCK_JACOBI, MHX_PREC_FP32 = 0, 1, 2, 0x100
+MHX_PREC_NONE, MHX_PREC_JACOBI, MHX_PREC_BLOCK_JACOBI, MHX_PREC_AMG, MHX_PREC_FP32 = 0, 1, 2, 3, 0x100
 MHX_MAX_MODES = 8
 MHX_MAX_NQ_MODES = 4
 MHX_RNG_UNIFORM, MHX_RNG_LOGNORMAL, MHX_RNG_NORMAL = 0, 1, 2

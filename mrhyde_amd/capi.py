@@ -61,6 +61,8 @@ def _load() -> C.CDLL:
         "mhx_server_info": (C.c_int, [vp, vp]),
         "mhx_server_last_error": (C.c_char_p, [vp]),
         "mhx_server_destroy": (None, [vp]),
+        "mhx_amg_setup": (C.c_int, [vp, vp, i32]),
+        "mhx_amg_level": (C.c_int, [vp, i32, i32, C.POINTER(i64), C.POINTER(C.c_double), vp, vp, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -79,5 +81,5 @@ EXPORTED = [
     "mhx_synchronize", "mhx_set_stream", "mhx_set_option", "mhx_get_option", "mhx_copy_probe", "mhx_batch_create", "mhx_batch_solve",
     "mhx_batch_results", "mhx_batch_info", "mhx_batch_set_wg", "mhx_batch_destroy","mhx_state_ptr", "mhx_state_upload", "mhx_state_download", "mhx_rng_fill", "mhx_version",
     "mhx_footprint", "mhx_ctx_update", "mhx_server_create", "mhx_server_submit", "mhx_server_poll",
-    "mhx_server_info", "mhx_server_last_error", "mhx_server_destroy",
+    "mhx_server_info", "mhx_server_last_error", "mhx_server_destroy", "mhx_amg_setup", "mhx_amg_level",
 ]

@@ -507,6 +507,43 @@ int mhx_linear_solve(mhx_ctx* ctx, const double* jac_vals, const double* r, doub
   MHX_GUARD_END(ctx)
 }
 
+int mhx_amg_setup(mhx_ctx* ctx, const double* jac_vals, int32_t flags) {
+  MHX_GUARD_BEGIN
+  MHX_REQUIRE(ctx, MHX_ERR_ARG, "null argument");
+  const double* A;
+  DevBuf<double> jstage;
+  if (jac_vals) {
+    if (flags & MHX_DEVICE_PTRS) {
+      A = jac_vals;
+    } else {
+      jstage.alloc(ctx->nnz);
+      MHX_HIP(hipMemcpyAsync(jstage.p, jac_vals, ctx->nnz * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+      A = jstage.p;
+    }
+  } else {
+    MHX_REQUIRE(ctx->jac_valid && ctx->jac.p, MHX_ERR_STATE, "no Jacobian held by the context");
+    A = ctx->jac.p;
+  }
+  amg_setup(*ctx, A, false);
+  MHX_HIP(hipStreamSynchronize(ctx->stream));
+  return amg_levels(*ctx);
+  MHX_GUARD_END(ctx)
+}
+
+int mhx_amg_level(mhx_ctx* ctx, int32_t level, int32_t which, int64_t info[4], double* lambda, int32_t* rp,
+                  int32_t* ci, double* vals) {
+  MHX_GUARD_BEGIN
+  MHX_REQUIRE(ctx && info, MHX_ERR_ARG, "null argument");
+  MHX_REQUIRE(amg_levels(*ctx) > 0, MHX_ERR_STATE, "no AMG hierarchy held by the context");
+  MHX_REQUIRE(!(level == 0 && which == 0 && vals), MHX_ERR_ARG,
+              "the level-0 operator is the caller's Jacobian; its values are not kept");
+  amg_download(*ctx, level, which, &info[0], &info[1], &info[2], rp, ci, vals);
+  info[3] = amg_levels(*ctx);
+  if (lambda) *lambda = amg_lambda(*ctx, level);
+  return MHX_OK;
+  MHX_GUARD_END(ctx)
+}
+
 int mhx_nonlinear_solve(mhx_ctx* ctx, double* u, const mhx_newton_opts* opts, mhx_solve_stats* stats) {
   MHX_GUARD_BEGIN
   MHX_REQUIRE(ctx && u && opts, MHX_ERR_ARG, "null argument");
@@ -635,6 +672,13 @@ int mhx_footprint(const mhx_problem_desc* d, const mhx_krylov_opts* opts, int64_
   int64_t bytes = 12 * Z + 4 * (N + 1) + N * (8 * 4 + 8 * 3 + 8 + 1 + 4) + 8 * 5 * N + 24 * nv + 16 * ne + (1 << 20);
   if (opts && opts->solver != MHX_KRYLOV_CG) bytes += 8 * N * ((opts->restart > 0 ? opts->restart : 50) + 3);
   if (opts && (opts->precond & MHX_PREC_FP32)) bytes += 4 * Z + 4 * 8 * N;
+  if (opts && (opts->precond & 0xff) == MHX_PREC_AMG) {
+    // fine level: A P in 27-slot boxes per (row, field) during set-up, P and
+    // R (at most 8 nf entries per fine row each), five level vectors and the
+    // power-iteration pair; coarse levels shrink by 27x or more each
+    const int64_t fine = 8 * 27 * N * nf + 2 * 12 * 8 * N * nf + 8 * 7 * N;
+    bytes += fine + fine / 8;
+  }
   if (n_dof) *n_dof = N;
   if (nnz) *nnz = Z;
   if (device_bytes) *device_bytes = bytes;

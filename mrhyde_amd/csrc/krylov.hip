@@ -15,6 +15,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 #include "mhx_internal.hpp"
@@ -353,6 +354,7 @@ struct Prec {
   int kind = MHX_PREC_NONE;
   int bs = 1;
   const T* d = nullptr;  // 1/diag (point) or BS x BS inverses (block)
+  const T* A = nullptr;  // AMG: the level-0 operator the V-cycle smooths with
 };
 
 template <class T>
@@ -375,6 +377,12 @@ static Prec<T> build_prec(Ctx& c, const double* A, int kind, KrylovVecs<T>& v) {
       case 4: hipLaunchKernelGGL((k_block_inv<4, T>), grid1(nblk), dim3(256), 0, c.stream, nblk, c.rowptr.p, c.colidx.p, A, v.prec.p); break;
       default: throw Error{MHX_ERR_UNSUPPORTED, "block Jacobi: block size must be 1..4"};
     }
+  } else if (kind == MHX_PREC_AMG) {
+    constexpr bool f32 = std::is_same<T, float>::value;
+    amg_setup(c, A, f32);
+    if constexpr (f32) P.A = c.kw.jac32.p;
+    else P.A = A;
+    return P;
   }
   MHX_HIP(hipGetLastError());
   P.d = v.prec.p;
@@ -393,6 +401,8 @@ static void prec_apply(Ctx& c, const Prec<T>& P, const T* v, T* out) {
       case 3: hipLaunchKernelGGL((k_block_apply<3, T>), grid1(n), dim3(256), 0, c.stream, n, P.d, v, out); break;
       default: hipLaunchKernelGGL((k_block_apply<4, T>), grid1(n), dim3(256), 0, c.stream, n, P.d, v, out); break;
     }
+  } else if (P.kind == MHX_PREC_AMG) {
+    amg_apply(c, P.A, v, out);
   } else {
     MHX_HIP(hipMemcpyAsync(out, v, n * sizeof(T), hipMemcpyDeviceToDevice, c.stream));
   }
@@ -404,7 +414,7 @@ template <class T>
 static void op_apply(Ctx& c, const T* A, const Prec<T>& P, const T* v, T* w, T* tmp) {
   if (P.kind == MHX_PREC_JACOBI) {
     spmv_t<T, T>(c, A, P.d, v, w);
-  } else if (P.kind == MHX_PREC_BLOCK_JACOBI) {
+  } else if (P.kind == MHX_PREC_BLOCK_JACOBI || P.kind == MHX_PREC_AMG) {
     prec_apply<T>(c, P, v, tmp);
     spmv_t<T, T>(c, A, nullptr, tmp, w);
   } else {
@@ -490,7 +500,7 @@ static int cg_core(Ctx& c, const T* A, const Prec<T>& P, const T* b, T* x, doubl
       ScopedPhase ph(c.prof, "Krylov::cg_window", c.stream);
       for (int k = 0; k < w; ++k) {
         const int cur = (it + k) & 1;
-        if (P.kind == MHX_PREC_BLOCK_JACOBI) {
+        if (P.kind == MHX_PREC_BLOCK_JACOBI || P.kind == MHX_PREC_AMG) {
           prec_apply<T>(c, P, v.r.p, v.z.p);
           hipLaunchKernelGGL(k_dot<T>, dim3(nb), dim3(256), 0, c.stream, n, v.r.p, v.z.p, c.kw.part.p, 0);
         } else {
@@ -736,7 +746,7 @@ static int mixed_solve(Ctx& c, const double* A, const double* b, double* x, cons
 int krylov_solve(Ctx& c, const double* A, const double* b, double* x, const mhx_krylov_opts& o, mhx_solve_stats* st) {
   ScopedPhase ph(c.prof, "LinearAlgebraInterface::linearSolver", c.stream);
   const int pk = o.precond & 0xff;
-  MHX_REQUIRE(pk >= MHX_PREC_NONE && pk <= MHX_PREC_BLOCK_JACOBI && (o.precond & ~(0xff | MHX_PREC_FP32)) == 0,
+  MHX_REQUIRE(pk >= MHX_PREC_NONE && pk <= MHX_PREC_AMG && (o.precond & ~(0xff | MHX_PREC_FP32)) == 0,
               MHX_ERR_ARG, "unknown preconditioner");
   MHX_REQUIRE(o.solver == 0 || o.solver == MHX_KRYLOV_CG || o.solver == MHX_KRYLOV_GMRES, MHX_ERR_ARG,
               "unknown Krylov solver");

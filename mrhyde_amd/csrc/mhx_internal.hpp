@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -179,6 +180,8 @@ struct KrylovWork {
 // ---------------------------------------------------------------------------
 // context
 // ---------------------------------------------------------------------------
+struct Amg;  // smoothed-aggregation hierarchy (amg.hip)
+
 struct Ctx {
   mhx_problem_desc desc{};
   BoxMesh mesh{};
@@ -206,6 +209,7 @@ struct Ctx {
   int spmv_lanes = 16;
   int nfield = 1;  // unknowns per node (consecutive DOFs): block size of block Jacobi
   int scatter_mode = kElementColouring;
+  std::shared_ptr<Amg> amg;  // built by a solve with the AMG preconditioner, kept for reuse of its buffers
   std::string err;
 };
 
@@ -233,5 +237,16 @@ int krylov_solve(Ctx& c, const double* A, const double* b, double* x, const mhx_
                  mhx_solve_stats* st);
 void set_dirichlet_dev(Ctx& c, double* u);
 void axpy(Ctx& c, long n, double a, const double* x, double* y);
+
+// smoothed-aggregation AMG (amg.hip): set-up from the FP64 operator A on the
+// context's pattern (and FP32 copies when fp32), and one V-cycle out = M^-1 v
+// with level-0 operator A (the FP64 A, or its FP32 copy)
+void amg_setup(Ctx& c, const double* A, bool fp32);
+void amg_apply(Ctx& c, const double* A, const double* v, double* out);
+void amg_apply(Ctx& c, const float* A, const float* v, float* out);
+int amg_levels(const Ctx& c);
+double amg_lambda(const Ctx& c, int level);
+void amg_download(Ctx& c, int level, int which, int64_t* rows, int64_t* cols, int64_t* nnz, int32_t* rp, int32_t* ci,
+                  double* vals);
 
 }  // namespace mhx

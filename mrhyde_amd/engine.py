@@ -62,7 +62,7 @@ def krylov_opts(solver: str = "gmres", tol: float = 1e-10, max_iters: int = 1000
     o = capi.KrylovOpts()
     o.solver = capi.MHX_KRYLOV_CG if solver.lower() in ("cg", "pcg", "pseudo block cg") else capi.MHX_KRYLOV_GMRES
     p = precond.lower()
-    o.precond = {"none": 0, "jacobi": 1, "block jacobi": 2}[p.replace("-fp32", "").strip()]
+    o.precond = {"none": 0, "jacobi": 1, "block jacobi": 2, "amg": 3, "muelu": 3}[p.replace("-fp32", "").strip()]
     if p.endswith("-fp32"):
         o.precond |= capi.MHX_PREC_FP32
     o.max_iters, o.restart, o.tol = int(max_iters), int(restart), float(tol)
@@ -243,6 +243,37 @@ class Mhx:
         if rc < 0 and rc != capi.MHX_ERR_NOT_CONVERGED:
             self._check(rc, "linearSolver")
         return x, st.as_dict()
+
+    def buildAMGPreconditioner(self, jac=None) -> int:
+        """Smoothed-aggregation hierarchy of J (None: the context's Jacobian),
+        kept in the context; returns the number of levels."""
+        pj, dj = _ptr(jac) if jac is not None else (None, False)
+        return self._check(lib.mhx_amg_setup(self._h, pj, capi.MHX_DEVICE_PTRS if dj else 0), "amg_setup")
+
+    def amg_level(self, level: int, which: str = "A"):
+        """One operator of the held hierarchy: which = "A", "P", "R" as scipy
+        CSR matrices (level-0 "A" as its pattern only), or "Cinv" (dense
+        inverse of the coarsest operator).  Returns (matrix, info) with info
+        = {rows, cols, nnz, levels, lambda}."""
+        import scipy.sparse as sp
+
+        w = {"A": 0, "P": 1, "R": 2, "Cinv": 3}[which]
+        info = (C.c_int64 * 4)()
+        lam = C.c_double()
+        self._check(lib.mhx_amg_level(self._h, level, w, info, C.byref(lam), None, None, None), "amg_level")
+        rows, cols, nnz = info[0], info[1], info[2]
+        meta = {"rows": rows, "cols": cols, "nnz": nnz, "levels": info[3], "lambda": lam.value}
+        if w == 3:
+            vals = np.empty(nnz)
+            self._check(lib.mhx_amg_level(self._h, level, w, info, None, None, None, vals.ctypes.data), "amg_level")
+            return vals.reshape(rows, cols), meta
+        rp = np.empty(rows + 1, dtype=np.int32)
+        ci = np.empty(nnz, dtype=np.int32)
+        vals = np.ones(nnz)
+        pv = None if (level == 0 and w == 0) else vals.ctypes.data
+        self._check(lib.mhx_amg_level(self._h, level, w, info, None, rp.ctypes.data, ci.ctypes.data, pv),
+                    "amg_level")
+        return sp.csr_matrix((vals, ci, rp), shape=(rows, cols)), meta
 
     def nonlinearSolver(self, u, max_iters: int = 10, nl_tol: float = 1e-6, backtracking: bool = True,
                         linear: Optional[capi.KrylovOpts] = None, linear_max_iters: int = 1000, **kw):
