@@ -40,7 +40,12 @@
 namespace mhx {
 
 constexpr int kAmgMaxLevels = 10;      // MueLu "max levels"
-constexpr long kAmgCoarseMax = 500;    // "coarse: max size"
+// MueLu's "coarse: max size" is 500 with a sparse direct coarse solver; the
+// coarsest solve here is a dense inverse built by one workgroup in O(n^3), so
+// the hierarchy coarsens on until at most 128 rows remain (one level more at
+// most, since a lattice level shrinks 27x or more per step in 3-D)
+constexpr long kAmgCoarseMax = 128;
+constexpr long kAmgDenseMax = 512;     // Gauss-Jordan kernel limit (2n columns <= 1024 threads)
 constexpr int kAmgPowerIters = 10;     // "sa: eigenvalue estimate num iterations"
 constexpr double kAmgDamping = 4.0 / 3.0;
 constexpr double kChebRatio = 7.0;     // "chebyshev: ratio eigenvalue"
@@ -502,9 +507,13 @@ __global__ __launch_bounds__(1024) void k_amg_gauss_jordan(int n, double* __rest
     for (int j = t; j < w; j += 1024) M[(long)col * w + j] *= inv;
     for (int r = t; r < n; r += 1024) F[r] = r == col ? 0.0 : M[(long)r * w + col];
     __syncthreads();
-    for (long idx = t; idx < (long)n * w; idx += 1024) {
-      const long r = idx / w, j = idx - r * w;
-      if (r != col) M[idx] = fma(-F[r], M[(long)col * w + j], M[idx]);
+    // thread t owns column t (w <= 1024); left-half columns before the pivot
+    // are already unit vectors with a zero in the pivot row, so they stay
+    if (t < w && (t >= col)) {
+      const double pc = M[(long)col * w + t];
+      if (pc != 0.0)
+        for (int r = 0; r < n; ++r)
+          if (r != col) M[(long)r * w + t] = fma(-F[r], pc, M[(long)r * w + t]);
     }
     __syncthreads();
   }
@@ -594,9 +603,11 @@ __global__ __launch_bounds__(256) void k_amg_spmv(long n, const int32_t* __restr
   if (l == 0) y[row] = (T)(add ? (double)y[row] + v : v);
 }
 
-// x = Cinv b, one wavefront per row of the dense coarsest inverse
+// x = Cinv b, one wavefront per row of the dense coarsest inverse.  The
+// inverse stays FP64 in both precisions: in FP32 its rounding error would be
+// amplified by the condition number of the coarsest operator.
 template <class T>
-__global__ __launch_bounds__(256) void k_amg_gemv(long n, const T* __restrict__ Cinv, const T* __restrict__ b,
+__global__ __launch_bounds__(256) void k_amg_gemv(long n, const double* __restrict__ Cinv, const T* __restrict__ b,
                                                   T* __restrict__ x) {
   const long row = ((long)blockIdx.x * blockDim.x + threadIdx.x) / 64;
   const int l = threadIdx.x % 64;
@@ -635,8 +646,7 @@ struct AmgLevel {
 
 struct Amg {
   std::vector<std::unique_ptr<AmgLevel>> lv;
-  DevBuf<double> cinv;    // dense inverse of the coarsest operator
-  DevBuf<float> cinv32;
+  DevBuf<double> cinv;    // dense inverse of the coarsest operator (FP64 in both precisions)
   DevBuf<double> scratch;  // set-up temporaries
   DevBuf<int> err;
   const double* A0 = nullptr;  // level-0 operator of the last set-up
@@ -819,7 +829,7 @@ void amg_setup(Ctx& c, const double* A, bool fp32) {
   {
     AmgLevel& Z = *H.lv.back();
     const long n = Z.n;
-    MHX_REQUIRE(n <= 4096, MHX_ERR_UNSUPPORTED, "AMG: coarsest level too large for the dense solve");
+    MHX_REQUIRE(n <= kAmgDenseMax, MHX_ERR_UNSUPPORTED, "AMG: coarsest level too large for the dense solve");
     H.scratch.alloc(std::max<size_t>(H.scratch.n, (size_t)2 * n * n + n));
     hipLaunchKernelGGL(k_amg_dense, g1(n), dim3(256), 0, c.stream, n, Z.rp, Z.ci, Acur, H.scratch.p);
     hipLaunchKernelGGL(k_amg_gauss_jordan, dim3(1), dim3(1024), 0, c.stream, (int)n, H.scratch.p,
@@ -828,11 +838,6 @@ void amg_setup(Ctx& c, const double* A, bool fp32) {
     hipLaunchKernelGGL(k_amg_extract_inv<double>, g1(n * n), dim3(256), 0, c.stream, n, H.scratch.p, H.cinv.p);
     MHX_HIP(hipGetLastError());
     check_err(c, H, "coarsest");
-    if (fp32) {
-      H.cinv32.alloc(n * n);
-      hipLaunchKernelGGL((k_amg_cast<double, float>), g1(n * n), dim3(256), 0, c.stream, n * n, H.cinv.p,
-                         H.cinv32.p);
-    }
   }
   // FP32 copies of the operators
   for (int l = 0; l < nlev; ++l) {
@@ -902,7 +907,7 @@ static void cheb_step(Ctx& c, AmgLevel& V, const LvView<T>& W, const T* b, const
 }
 
 template <class T>
-static void vcycle(Ctx& c, Amg& H, const T* A0, const T* cinv, const T* v, T* out) {
+static void vcycle(Ctx& c, Amg& H, const T* A0, const double* cinv, const T* v, T* out) {
   const int nl = (int)H.lv.size();
   // downward: pre-smooth (zero start), residual, restrict
   for (int l = 0; l < nl - 1; ++l) {
@@ -954,11 +959,13 @@ static void vcycle(Ctx& c, Amg& H, const T* A0, const T* cinv, const T* v, T* ou
 
 void amg_apply(Ctx& c, const double* A, const double* v, double* out) {
   MHX_REQUIRE(c.amg && !c.amg->lv.empty() && !c.amg->fp32, MHX_ERR_STATE, "AMG: no FP64 hierarchy");
+  ScopedPhase ph(c.prof, "MueLu::Hierarchy::Iterate", c.stream);
   vcycle<double>(c, *c.amg, A, c.amg->cinv.p, v, out);
 }
 void amg_apply(Ctx& c, const float* A, const float* v, float* out) {
   MHX_REQUIRE(c.amg && !c.amg->lv.empty() && c.amg->fp32, MHX_ERR_STATE, "AMG: no FP32 hierarchy");
-  vcycle<float>(c, *c.amg, A, c.amg->cinv32.p, v, out);
+  ScopedPhase ph(c.prof, "MueLu::Hierarchy::Iterate", c.stream);
+  vcycle<float>(c, *c.amg, A, c.amg->cinv.p, v, out);
 }
 
 int amg_levels(const Ctx& c) { return c.amg ? (int)c.amg->lv.size() : 0; }

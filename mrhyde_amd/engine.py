@@ -244,10 +244,15 @@ class Mhx:
             self._check(rc, "linearSolver")
         return x, st.as_dict()
 
-    def buildAMGPreconditioner(self, jac=None) -> int:
-        """Smoothed-aggregation hierarchy of J (None: the context's Jacobian),
-        kept in the context; returns the number of levels."""
-        pj, dj = _ptr(jac) if jac is not None else (None, False)
+    def buildAMGPreconditioner(self, jac=None, resident: bool = False) -> int:
+        """Smoothed-aggregation hierarchy of J (None: the Jacobian of the last
+        Newton step; resident=True: the context's device buffer MHX_BUF_JAC
+        written by assemble_resident), kept in the context; returns the number
+        of levels."""
+        if resident:
+            pj, dj = C.c_void_p(self.state_ptr(capi.MHX_BUF_JAC)[0]), True
+        else:
+            pj, dj = _ptr(jac)
         return self._check(lib.mhx_amg_setup(self._h, pj, capi.MHX_DEVICE_PTRS if dj else 0), "amg_setup")
 
     def amg_level(self, level: int, which: str = "A"):

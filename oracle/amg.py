@@ -29,7 +29,7 @@ import numpy as np
 import scipy.sparse as sp
 
 MAX_LEVELS = 10
-COARSE_MAX = 500
+COARSE_MAX = 128  # the device coarsens to at most 128 rows before its dense coarsest solve
 POWER_ITERS = 10
 DAMPING = 4.0 / 3.0
 CHEB_RATIO = 7.0

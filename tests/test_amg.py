@@ -67,7 +67,7 @@ def oracle_hierarchy(name):
 @pytest.mark.parametrize("name", list(CASES))
 def test_oracle_hierarchy_matches_scipy_products(name):
     levels, r, j, A = oracle_hierarchy(name)
-    assert len(levels) >= 2 and levels[-1]["A"].shape[0] <= 500
+    assert len(levels) >= 2 and levels[-1]["A"].shape[0] <= 128
     p = CASES[name][0]
     fixed = np.zeros(A.shape[0], bool)
     from oracle import Oracle
