@@ -182,6 +182,35 @@ def ns_bench(n: int, steps: int, newton: bool = True):
     return out
 
 
+def c2_newton_bench(n: int):
+    """The nonlinear Q2 thermal problem solved end to end: Newton from u = 0
+    ("max nonlinear iters" 5, no backtracking), each linear solve GMRES(50) to
+    1e-10 with the smoothed-aggregation preconditioner (rebuilt per step, as
+    the reference's MueLu set-up with reuse "none").  Wall time from host u to
+    host u, per-phase times from the engine's timers."""
+    from mrhyde_amd import Mhx
+
+    g = Mhx(c2_problem(n))
+    u = g.setDirichlet(np.zeros(g.n_dof))
+    g.profile(True)
+    g.profile_reset()
+    t0 = time.perf_counter()
+    u, st = g.nonlinearSolver(u, max_iters=5, nl_tol=1e-12, backtracking=False, solver="gmres", restart=50,
+                              tol=1e-10, linear_max_iters=1000, precond="amg")
+    wall = time.perf_counter() - t0
+    prof = g.profile_read()
+    g.profile(False)
+    g.close()
+    ph = {k: round(v[0], 1) for k, v in prof.items()
+          if k in ("AssemblyManager::assembleJacRes", "LinearAlgebraInterface::linearSolver",
+                   "LinearAlgebraInterface::buildAMGPreconditioner", "MueLu::Hierarchy::Iterate")}
+    return {"workload": f"Newton on the {n}^3 Q2 nonlinear thermal problem from u = 0, 5 steps max, "
+                        "GMRES(50) + AMG to 1e-10 per step",
+            "wall_s": round(wall, 3), "nl_iters": st["nl_iters"], "krylov_iters": st["iters"],
+            "res_norm0": st["res_norm0"], "res_norm": st["res_norm"], "last_linear_rel_res": st["rel_res"],
+            "phases_ms": ph}
+
+
 def c5_requests(count: int, seed: int):
     """Thermal solve requests: with probability 1/2 a 2-D Q1 mesh of n^2
     elements, n log-uniform on [32, 256], else a 3-D mesh of n^3 elements, n
@@ -345,6 +374,7 @@ def main():
     ap.add_argument("--probe-gib", type=float, default=1.0, help="size of the plain-copy bandwidth probe")
     ap.add_argument("--skip-subgrid", action="store_true", help="skip the batched-subgrid measurement")
     ap.add_argument("--krylov-iters", type=int, default=100, help="GMRES iterations timed (0 = skip)")
+    ap.add_argument("--no-newton", action="store_true", help="skip the end-to-end Newton solve")
     ap.add_argument("--ns-n", type=int, default=64, help="Navier-Stokes cavity elements per axis (0 = skip)")
     ap.add_argument("--ns-no-newton", action="store_true", help="Navier-Stokes: assembly only")
     ap.add_argument("--serve-requests", type=int, default=240, help="serving requests (0 = skip)")
@@ -419,6 +449,7 @@ def main():
     cpu = None if args.no_cpu_baseline else step("cpu baseline", lambda: cpu_baseline(n))
     krylov = step("krylov", lambda: krylov_bench(g, args.krylov_iters)) if args.krylov_iters > 0 else None
     g.close()
+    newton = None if args.no_newton else step("newton", lambda: c2_newton_bench(n))
     steps_small = max(3, min(args.steps, 10))
     subgrid = None if args.skip_subgrid else step("subgrid", lambda: subgrid_bench(steps_small))
     ns = step("navier stokes", lambda: ns_bench(args.ns_n, steps_small, not args.ns_no_newton)) if args.ns_n > 0 else None
@@ -449,6 +480,7 @@ def main():
         "cpu_baseline": cpu,
         "phases_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in prof.items()},
         "krylov": krylov,
+        "newton": newton,
         "subgrid": subgrid,
         "navier_stokes": ns,
         "serving": serving,
