@@ -83,6 +83,9 @@ typedef struct mhx_problem_desc {
   int32_t n_modes;
   double  mode_amp[MHX_MAX_MODES];
   double  mode_freq[MHX_MAX_MODES][3];
+  /* "density" and "specific heat" (thermal.cpp:83-84), constants; 0 means
+   * the reference default 1.0.  They scale the time derivative only. */
+  double  th_density, th_specific_heat;
 
   /* Neumann conditions (thermal boundaryResidual, "Neumann",
    * src/physics/thermal.cpp:210-222): on the sides in neumann_sides the
@@ -230,6 +233,19 @@ int  mhx_amg_setup(mhx_ctx* ctx, const double* jac_vals, int32_t flags);
  * Introspection of MueLu::Hierarchy levels; no single reference call. */
 int  mhx_amg_level(mhx_ctx* ctx, int32_t level, int32_t which, int64_t info[4], double* lambda, int32_t* rp,
                    int32_t* ci, double* vals);
+
+/* Time-integration state used by every following assembly and Newton solve
+ * (thermal): the residual gains rho cp dT/dt phi with, per node,
+ *   dT/dt = (BDF_0 u + sum_{k=1..order} BDF_k u_prev[k-1]) / dt,
+ * the backward-difference weights of `bdf_order` 1..6
+ * (SolverManager::setBackwardDifference, solverManager_setup.hpp:443-500)
+ * and the single-stage BWE tableau (Workset::computeSolnTransientSeeded,
+ * src/tools/workset.cpp:601-680); the Jacobian gains BDF_0/dt rho cp times
+ * the mass matrix.  u_prev: bdf_order vectors of n_dof values, most recent
+ * first (MHX_DEVICE_PTRS in flags: device pointer).  The source is evaluated
+ * from the descriptor's current amplitudes (update them per step with
+ * mhx_ctx_update).  bdf_order = 0 returns to steady assembly. */
+int  mhx_set_time_state(mhx_ctx* ctx, double dt, int32_t bdf_order, const double* u_prev, int32_t flags);
 
 /* Newton on u in place (u must already carry the Dirichlet values, see
  * mhx_set_dirichlet).  Replaces SolverManager::nonlinearSolver

@@ -49,6 +49,7 @@ class ProblemDesc(C.Structure):
         ("n_modes", C.c_int32),
         ("mode_amp", C.c_double * MHX_MAX_MODES),
         ("mode_freq", (C.c_double * 3) * MHX_MAX_MODES),
+        ("th_density", C.c_double), ("th_specific_heat", C.c_double),
         ("neumann_sides", C.c_uint32), ("n_nq_modes", C.c_int32),
         ("nq_amp", (C.c_double * 3) * MHX_MAX_NQ_MODES), ("nq_freq", (C.c_double * 3) * MHX_MAX_NQ_MODES),
         ("nq_phase", (C.c_double * 3) * MHX_MAX_NQ_MODES),
@@ -101,6 +102,16 @@ class BatchDesc(C.Structure):
         ("tol", C.c_double), ("max_iters", C.c_int32), ("precision", C.c_int32), ("inner_tol", C.c_double),
         ("max_refine", C.c_int32), ("wg_size", C.c_int32), ("use_graph", C.c_int32), ("reserved", C.c_int32 * 4),
     ]
+
+
+def bdf_schedule(order: int, nsteps: int, startup_order=None, startup_steps=None):
+    """BDF order of each time step as SolverManager sets it
+    (solverManager_construct.hpp:145-194, solverManager_solvers.hpp:93-97):
+    "transient startup BDF order" (default: the order itself) for the first
+    "transient startup steps" (default: the order) steps, then the order."""
+    so = order if startup_order is None else int(startup_order)
+    ss = order if startup_steps is None else int(startup_steps)
+    return [so if (order > 1 and step < ss) else order for step in range(nsteps)]
 
 
 def subgrid_workload(n_inst: int = 4096, seed: int = 4096, sizes=(4, 6, 8, 12, 16), sigma: float = 1.0):
@@ -156,6 +167,8 @@ class Problem:
     diff_u2: float = 0.0
     src_elem: Optional[np.ndarray] = None
     modes: Sequence = ()
+    density: float = 1.0         # thermal "density" (scales dT/dt)
+    specific_heat: float = 1.0   # thermal "specific heat"
     # Neumann sides and flux field q(x): terms (amp[3], freq[3], phase[3]),
     # q_c = sum amp_c prod_d sin(freq_d x_d + phase_d); g = q . n
     neumann_sides: int = 0
@@ -228,6 +241,7 @@ class Problem:
             fr = list(freq) + [0.0] * (3 - len(freq))
             for a in range(3):
                 d.mode_freq[m][a] = float(fr[a])
+        d.th_density, d.th_specific_heat = float(self.density), float(self.specific_heat)
         if len(self.neumann_flux) > MHX_MAX_NQ_MODES:
             raise ValueError("too many Neumann flux terms")
         d.neumann_sides, d.n_nq_modes = int(self.neumann_sides), len(self.neumann_flux)

@@ -339,7 +339,10 @@ __device__ __forceinline__ void row_range(int c, int p, int n, int& lo, int& len
   }
 }
 
-template <int DIM, int P, int NQ1, bool JAC, int MODE>
+// TR: transient, the source row gains rho cp (alpha_t u + h(x)) with h the
+// nodal history term (workset.cpp:601-680 seeds u_dot = alpha_t u + beta_t
+// per node; thermal.cpp:129 multiplies it by rho cp)
+template <int DIM, int P, int NQ1, bool JAC, int MODE, bool TR = false>
 __global__ __launch_bounds__(256) void k_thermal(AsmArgs a) {
   using C = Cfg<DIM, P, NQ1>;
   constexpr int NB = C::NB, NQ = C::NQ, NV = C::NV, NC = C::NC, G = C::G, GPB = C::GPB, P1 = C::P1;
@@ -350,9 +353,10 @@ __global__ __launch_bounds__(256) void k_thermal(AsmArgs a) {
   __shared__ double s_sres[SEG ? SG::kRows : 1];
   __shared__ int s_srp[SEG ? SG::kRows : 1];
   // rows of Dhat that can be nonzero: the flux rows, plus the source row when
-  // the source depends on the solution
-  constexpr int NR = ThermalPhysics::kSrcDependsOnU ? NC : DIM;
+  // the source depends on the solution (always when transient)
+  constexpr int NR = (ThermalPhysics::kSrcDependsOnU || TR) ? NC : DIM;
   __shared__ double s_u[GPB][NB];
+  __shared__ double s_h[TR ? GPB : 1][TR ? NB : 1];
   __shared__ double s_X[GPB][NV * DIM];
   // Dhat = w T D T^T per point
   constexpr int NW = JAC ? NQ * NR * NC : 1;
@@ -388,7 +392,11 @@ __global__ __launch_bounds__(256) void k_thermal(AsmArgs a) {
 
   // 1. gather --------------------------------------------------------------
   if (active) {
-    for (int j = lane; j < NB; j += G) s_u[g][j] = a.u[node_of<DIM, P>(a.m, ec, j)];
+    for (int j = lane; j < NB; j += G) {
+      const long nd = node_of<DIM, P>(a.m, ec, j);
+      s_u[g][j] = a.u[nd];
+      if constexpr (TR) s_h[g][j] = a.hist[nd];
+    }
     for (int t = lane; t < NV * DIM; t += G) {
       int v = t / DIM, d = t % DIM;
       long vid = (long)(ec[0] + (v & 1)) +
@@ -404,8 +412,8 @@ __global__ __launch_bounds__(256) void k_thermal(AsmArgs a) {
     const double k0 = a.coef.k0 ? a.coef.k0[e] : 1.0;
     for (int q = lane; q < NQ; q += G) {
       const int qi[3] = {q % NQ1, (q / NQ1) % NQ1, DIM == 3 ? q / (NQ1 * NQ1) : 0};
-      // reference value / gradient of u
-      double uq = 0.0, gu[DIM];
+      // reference value / gradient of u (and the history term's value)
+      double uq = 0.0, gu[DIM], hq = 0.0;
 #pragma unroll
       for (int c = 0; c < DIM; ++c) gu[c] = 0.0;
       // tensor-product evaluation: the outer basis axis runs as a loop (its
@@ -428,11 +436,13 @@ __global__ __launch_bounds__(256) void k_thermal(AsmArgs a) {
           const double uj = s_u[g][ji + NI * jo];
           if constexpr (DIM == 2) {
             uq = fma(uj, bq[0][jx] * bo, uq);
+            if constexpr (TR) hq = fma(s_h[g][ji + NI * jo], bq[0][jx] * bo, hq);
             gu[0] = fma(uj, dq[0][jx] * bo, gu[0]);
             gu[1] = fma(uj, bq[0][jx] * dox, gu[1]);
           } else {
             const double bxy = bq[0][jx] * bq[1][jy];
             uq = fma(uj, bxy * bo, uq);
+            if constexpr (TR) hq = fma(s_h[g][ji + NI * jo], bxy * bo, hq);
             gu[0] = fma(uj, dq[0][jx] * bq[1][jy] * bo, gu[0]);
             gu[1] = fma(uj, bq[0][jx] * dq[1][jy] * bo, gu[1]);
             gu[2] = fma(uj, bxy * dox, gu[2]);
@@ -516,6 +526,11 @@ __global__ __launch_bounds__(256) void k_thermal(AsmArgs a) {
       for (int c = 0; c < DIM; ++c) Gd[c] = qd_seed<NC>(gp[c], c);
       QDual<NC> flux[DIM], src;
       ThermalPhysics::volume<DIM>(k0, a.coef.diff_u2, f, U, Gd, flux, src);
+      if constexpr (TR) {
+        // + rho cp dT/dt, dT/dt = alpha_t U + h(x): value and d/du slot
+        src.v += a.rho_cp * fma(a.alpha_t, uq, hq);
+        src.d[DIM] += a.rho_cp * a.alpha_t;
+      }
       // Fhat = w T (flux, src);  Dhat = w T D T^T,  T = diag(J^-1, 1)
       double* F = &s_F[g][q * NC];
 #pragma unroll
@@ -916,7 +931,8 @@ template <int DIM, int P, int NQ1>
 static void launch_colors(Ctx& c, AsmArgs& a, bool jac) {
   using C = Cfg<DIM, P, NQ1>;
   const int ncol = 1 << DIM;
-  const int mode = c.scatter_mode;
+  // the transient variant is built for element colouring only
+  const int mode = a.hist ? (int)kElementColouring : c.scatter_mode;
   a.gpb = C::GPB;
   a.blocked = mode;
   a.nsub = mode == kBlockColouring ? ncol : 1;
@@ -949,6 +965,9 @@ static void launch_colors(Ctx& c, AsmArgs& a, bool jac) {
     } else if (mode == kBlockColouring) {
       if (jac) hipLaunchKernelGGL((k_thermal<DIM, P, NQ1, true, kBlockColouring>), grid, block, 0, c.stream, a);
       else hipLaunchKernelGGL((k_thermal<DIM, P, NQ1, false, kBlockColouring>), grid, block, 0, c.stream, a);
+    } else if (a.hist) {
+      if (jac) hipLaunchKernelGGL((k_thermal<DIM, P, NQ1, true, kElementColouring, true>), grid, block, 0, c.stream, a);
+      else hipLaunchKernelGGL((k_thermal<DIM, P, NQ1, false, kElementColouring, true>), grid, block, 0, c.stream, a);
     } else {
       if (jac) hipLaunchKernelGGL((k_thermal<DIM, P, NQ1, true, kElementColouring>), grid, block, 0, c.stream, a);
       else hipLaunchKernelGGL((k_thermal<DIM, P, NQ1, false, kElementColouring>), grid, block, 0, c.stream, a);
@@ -999,6 +1018,13 @@ void assemble_thermal(Ctx& c, const double* u, double* res, double* jac, bool wa
   a.fixed = c.fixed.p;
   a.res = want_res ? res : nullptr;
   a.jac = want_jac ? jac : nullptr;
+  if (c.bdf_order > 0) {
+    a.hist = c.hist.p;
+    a.alpha_t = c.alpha_t;
+    const double rho = c.desc.th_density > 0.0 ? c.desc.th_density : 1.0;
+    const double cp = c.desc.th_specific_heat > 0.0 ? c.desc.th_specific_heat : 1.0;
+    a.rho_cp = rho * cp;
+  }
   // Overwrite mode: the first scheduled element touching an entry writes it,
   // which also writes the Dirichlet rows (0, diagonal 1).  Accumulate mode:
   // contributions are added and the Dirichlet diagonal is set afterwards.

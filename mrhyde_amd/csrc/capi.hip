@@ -70,6 +70,8 @@ static void validate(const mhx_problem_desc* d) {
               "empty box");
   MHX_REQUIRE(d->n_modes >= 0 && d->n_modes <= MHX_MAX_MODES, MHX_ERR_ARG, "n_modes out of range");
   MHX_REQUIRE(d->n_nq_modes >= 0 && d->n_nq_modes <= MHX_MAX_NQ_MODES, MHX_ERR_ARG, "n_nq_modes out of range");
+  MHX_REQUIRE(d->th_density >= 0.0 && d->th_specific_heat >= 0.0, MHX_ERR_ARG,
+              "density and specific heat must be >= 0 (0 = 1.0)");
   MHX_REQUIRE(d->neumann_sides == 0 || d->physics == MHX_PHYS_THERMAL, MHX_ERR_UNSUPPORTED,
               "Neumann sides are built for the thermal module only");
   if (d->physics == MHX_PHYS_NAVIER_STOKES) {
@@ -504,6 +506,47 @@ int mhx_linear_solve(mhx_ctx* ctx, const double* jac_vals, const double* r, doub
   MHX_HIP(hipStreamSynchronize(ctx->stream));
   if (stats) stats->status = s;
   return s;
+  MHX_GUARD_END(ctx)
+}
+
+int mhx_set_time_state(mhx_ctx* ctx, double dt, int32_t bdf_order, const double* u_prev, int32_t flags) {
+  MHX_GUARD_BEGIN
+  MHX_REQUIRE(ctx, MHX_ERR_ARG, "null argument");
+  Ctx& c = *ctx;
+  MHX_REQUIRE(bdf_order >= 0 && bdf_order <= 6, MHX_ERR_ARG, "bdf_order must be 0..6");
+  if (bdf_order == 0) {
+    c.bdf_order = 0;
+    c.alpha_t = 0.0;
+    return MHX_OK;
+  }
+  MHX_REQUIRE(c.desc.physics == MHX_PHYS_THERMAL, MHX_ERR_UNSUPPORTED, "time integration is built for the thermal module");
+  MHX_REQUIRE(dt > 0.0 && std::isfinite(dt), MHX_ERR_ARG, "dt must be positive");
+  MHX_REQUIRE(u_prev, MHX_ERR_ARG, "null history");
+  // backward-difference weights without 1/dt (solverManager_setup.hpp:455-500)
+  static const double W[7][7] = {
+      {0}, {1.0, -1.0}, {1.5, -2.0, 0.5}, {11.0 / 6.0, -3.0, 1.5, -1.0 / 3.0},
+      {25.0 / 12.0, -4.0, 3.0, -4.0 / 3.0, 1.0 / 4.0},
+      {137.0 / 60.0, -5.0, 5.0, -10.0 / 3.0, 75.0 / 60.0, -1.0 / 5.0},
+      {147.0 / 60.0, -6.0, 15.0 / 2.0, -20.0 / 3.0, 225.0 / 60.0, -72.0 / 60.0, 1.0 / 6.0}};
+  const long n = c.mesh.ndof;
+  const double* prev = u_prev;
+  DevBuf<double> stage;
+  if (!(flags & MHX_DEVICE_PTRS)) {
+    stage.alloc((size_t)n * bdf_order);
+    MHX_HIP(hipMemcpyAsync(stage.p, u_prev, (size_t)n * bdf_order * sizeof(double), hipMemcpyHostToDevice,
+                           c.stream));
+    prev = stage.p;
+  }
+  c.hist.alloc(n);
+  MHX_HIP(hipMemsetAsync(c.hist.p, 0, n * sizeof(double), c.stream));
+  // hist = sum_k BDF_k u_prev[k-1] / dt, summed in k order
+  const double timewt = 1.0 / dt;
+  for (int k = 1; k <= bdf_order; ++k) axpy(c, n, W[bdf_order][k] * timewt, prev + (long)(k - 1) * n, c.hist.p);
+  c.alpha_t = W[bdf_order][0] * timewt;
+  c.bdf_order = bdf_order;
+  c.jac_valid = false;
+  MHX_HIP(hipStreamSynchronize(c.stream));
+  return MHX_OK;
   MHX_GUARD_END(ctx)
 }
 

@@ -89,6 +89,10 @@ struct AsmArgs {
   int s[3];               // block colouring: groups per axis (prod = gpb)
   int bw[3];              // block colouring: block width in elements (2*s)
   int first_touch;        // overwrite on first touch instead of accumulate
+  // transient (backward differences): dT/dt = alpha_t u + sum_j hist_j phi_j
+  const double* hist;     // [ndof] nodal sum_k BDF_k u_prev[k-1] / dt, or nullptr (steady)
+  double alpha_t;         // BDF_0 / dt
+  double rho_cp;          // density * specific heat
 };
 
 enum ScatterMode { kElementColouring = 0, kBlockColouring = 1, kSegmentMerge = 2 };
@@ -210,6 +214,10 @@ struct Ctx {
   int nfield = 1;  // unknowns per node (consecutive DOFs): block size of block Jacobi
   int scatter_mode = kElementColouring;
   std::shared_ptr<Amg> amg;  // built by a solve with the AMG preconditioner, kept for reuse of its buffers
+  // time-integration state (mhx_set_time_state); bdf_order 0 = steady
+  int bdf_order = 0;
+  double alpha_t = 0.0;
+  DevBuf<double> hist;       // [ndof] nodal history term of dT/dt
   std::string err;
 };
 

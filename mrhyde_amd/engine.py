@@ -21,7 +21,7 @@ from typing import Optional
 import numpy as np
 
 from . import capi
-from .abi import Problem, rng_fill  # noqa: F401
+from .abi import Problem, bdf_schedule, rng_fill  # noqa: F401
 from .capi import lib
 
 
@@ -297,6 +297,58 @@ class Mhx:
         if rc < 0 and rc != capi.MHX_ERR_NOT_CONVERGED:
             self._check(rc, "nonlinearSolver")
         return u, st.as_dict()
+
+    def update(self, problem: Problem):
+        """New coefficients on the same mesh (mhx_ctx_update): k0, sources,
+        density / specific heat, Dirichlet values."""
+        self.problem = problem
+        self._desc = problem.desc()
+        self._check(lib.mhx_ctx_update(self._h, C.byref(self._desc)), "update")
+
+    def set_time_state(self, dt: float, order: int, u_prev=()):
+        """Backward-difference state for the following assemblies and solves
+        (u_prev: `order` vectors, most recent first; order 0 = steady)."""
+        if order == 0:
+            self._check(lib.mhx_set_time_state(self._h, 0.0, 0, None, 0), "set_time_state")
+            return
+        if all(isinstance(v, np.ndarray) for v in u_prev[:order]):
+            buf = np.ascontiguousarray(np.concatenate([np.asarray(v, dtype=np.float64) for v in u_prev[:order]]))
+            self._check(lib.mhx_set_time_state(self._h, float(dt), int(order), buf.ctypes.data, 0),
+                        "set_time_state")
+        else:
+            import torch
+
+            buf = torch.cat([v.reshape(-1) for v in u_prev[:order]]).contiguous()
+            self._follow_torch_stream()
+            self._check(lib.mhx_set_time_state(self._h, float(dt), int(order), C.c_void_p(buf.data_ptr()),
+                                               capi.MHX_DEVICE_PTRS), "set_time_state")
+
+    def transientSolver(self, u0, dt: float, nsteps: int, order: int = 1, startup_order=None, startup_steps=None,
+                        t0: float = 0.0, at_time=None, record=None, **newton_kw):
+        """SolverManager::transientSolver (solverManager_solvers.hpp:40-170)
+        with the BWE tableau: history vectors start as zeros and shift each
+        step (u_prev[0] = u_n); the first `startup_steps` steps use
+        `startup_order` (bdf_schedule); at_time(t) -> Problem gives the
+        coefficients at the stage time t_n + dt; record(t, u) sees each step.
+        Keyword arguments go to nonlinearSolver (defaults: CG + Jacobi to
+        1e-12, Newton to 1e-12)."""
+        kw = dict(max_iters=10, nl_tol=1e-12, solver="cg", tol=1e-12, linear_max_iters=10000)
+        kw.update(newton_kw)
+        u = np.ascontiguousarray(u0, dtype=np.float64).copy()
+        hist = [np.zeros_like(u) for _ in range(max(order, startup_order or order))]
+        t = t0
+        for k in bdf_schedule(order, nsteps, startup_order, startup_steps):
+            hist = [u.copy()] + hist[:-1]
+            t = t + dt
+            if at_time is not None:
+                self.update(at_time(t))
+            self.set_time_state(dt, k, hist)
+            self.setDirichlet(u)
+            u, _ = self.nonlinearSolver(u, **kw)
+            if record is not None:
+                record(t, u)
+        self.set_time_state(dt, 0)
+        return u
 
     def computeError(self, u, modes):
         amp = np.array([m[0] for m in modes], dtype=np.float64)

@@ -167,6 +167,11 @@ struct Problem {
   std::vector<char> fixed;           // strong-DBC mask
   std::vector<double> dbcval;        // [ndof]
   int ad_size;
+  // transient state (Workset::computeSolnTransientSeeded, workset.cpp:601-680,
+  // BWE tableau: A = b = c = 1): u_dot_dof = alpha_t u_dof + beta_t[dof]
+  int bdf = 0;
+  double alpha_t = 0.0;
+  std::vector<double> beta_t;        // [ndof] sum_k BDF_k u_prev[k-1] / dt
 
   void vertex_of_elem(long e, int v, double* X) const;
   void elem_coords(long e, long* c) const {
@@ -454,21 +459,32 @@ static void assemble_thermal(const Problem& P, const double* u, bool want_jac, d
           for (int a = 0; a < dim; ++a) dT[q * dim + a] += uAD[le * nb + j] * G.grad[(j * nq + q) * dim + a];
         }
       }
-      // FunctionManager::evaluate("thermal source" / "thermal diffusion")
+      // time derivative field "T_t": the seeded nodal u_dot = alpha_t u + beta_t
+      // evaluated like T (zero when steady)
+      std::vector<EvalT> Tt(nq, EvalT(0.0));
+      if (P.bdf > 0) {
+        for (int j = 0; j < nb; ++j) {
+          EvalT ud = uAD[le * nb + j] * P.alpha_t + P.beta_t[P.lids[e * nb + j]];
+          for (int q = 0; q < nq; ++q) Tt[q] += ud * P.bref[j * nq + q];
+        }
+      }
+      // FunctionManager::evaluate("thermal source" / "thermal diffusion" /
+      // "density" / "specific heat")
       std::vector<double> source(nq);
       std::vector<EvalT> diff(nq);
+      const double rho = P.desc.th_density > 0.0 ? P.desc.th_density : 1.0;
+      const double cp = P.desc.th_specific_heat > 0.0 ? P.desc.th_specific_heat : 1.0;
       for (int q = 0; q < nq; ++q) {
         source[q] = source_at(P, e, &G.x[q * 3]);
         EvalT TT = T[q] * T[q];
         EvalT onep = 1.0 + TT * P.desc.diff_u2;
         diff[q] = P.k0[e] * (P.desc.diff_u2 != 0.0 ? onep : EvalT(1.0));
       }
-      // "Thermal volume resid 3D part 1" (steady: rho*cp*dTdt == 0)
+      // "Thermal volume resid 3D part 1": (rho cp dT/dt - source) w phi + diff grad T . grad phi w
       std::vector<EvalT> r(nb, EvalT(0.0));
       for (int i = 0; i < nb; ++i) {
         for (int q = 0; q < nq; ++q) {
-          EvalT zero(0.0);
-          r[i] += (zero - source[q]) * G.wts[q] * P.bref[i * nq + q];
+          r[i] += (Tt[q] * (rho * cp) - source[q]) * G.wts[q] * P.bref[i * nq + q];
           for (int a = 0; a < dim; ++a)
             r[i] += diff[q] * dT[q * dim + a] * G.wts[q] * G.grad[(i * nq + q) * dim + a];
         }
@@ -997,6 +1013,50 @@ int orc_linear_solve(void* h, const double* A, const double* b, double* x, const
 }
 int orc_newton(void* h, double* u, const mhx_newton_opts* o, mhx_solve_stats* st) {
   return orc::newton(*(orc::Problem*)h, u, o, st);
+}
+// Backward-difference state (SolverManager::setBackwardDifference,
+// solverManager_setup.hpp:443-500: the weights exclude 1/dt, which
+// Workset::computeSolnTransientSeeded applies with b(stage) = 1 for BWE).
+int orc_set_time_state(void* h, double dt, int32_t order, const double* u_prev) {
+  auto* P = (orc::Problem*)h;
+  static const double W[7][7] = {
+      {0}, {1.0, -1.0}, {1.5, -2.0, 0.5}, {11.0 / 6.0, -3.0, 1.5, -1.0 / 3.0},
+      {25.0 / 12.0, -4.0, 3.0, -4.0 / 3.0, 1.0 / 4.0},
+      {137.0 / 60.0, -5.0, 5.0, -10.0 / 3.0, 75.0 / 60.0, -1.0 / 5.0},
+      {147.0 / 60.0, -6.0, 15.0 / 2.0, -20.0 / 3.0, 225.0 / 60.0, -72.0 / 60.0, 1.0 / 6.0}};
+  if (order < 0 || order > 6 || (order > 0 && !(dt > 0.0))) return MHX_ERR_ARG;
+  P->bdf = order;
+  if (order == 0) {
+    P->beta_t.clear();
+    P->alpha_t = 0.0;
+    return MHX_OK;
+  }
+  const double timewt = 1.0 / dt;
+  P->alpha_t = W[order][0] * timewt;
+  P->beta_t.assign(P->ndof, 0.0);
+  for (long g = 0; g < P->ndof; ++g) {
+    double b = 0.0;
+    for (int s = 1; s <= order; ++s) b += W[order][s] * u_prev[(long)(s - 1) * P->ndof + g];
+    P->beta_t[g] = b * timewt;
+  }
+  return MHX_OK;
+}
+// coefficient update between steps: source modes, density, specific heat and
+// the constant Dirichlet value (the mesh and sides stay)
+int orc_update(void* h, const mhx_problem_desc* d) {
+  auto* P = (orc::Problem*)h;
+  P->desc.n_modes = d->n_modes;
+  std::copy(d->mode_amp, d->mode_amp + MHX_MAX_MODES, P->desc.mode_amp);
+  for (int m = 0; m < MHX_MAX_MODES; ++m)
+    for (int a = 0; a < 3; ++a) P->desc.mode_freq[m][a] = d->mode_freq[m][a];
+  P->desc.th_density = d->th_density;
+  P->desc.th_specific_heat = d->th_specific_heat;
+  if (!d->dbc_values) {
+    P->desc.dbc_const = d->dbc_const;
+    for (long g = 0; g < P->ndof; ++g)
+      if (P->fixed[g]) P->dbcval[g] = d->dbc_const;
+  }
+  return MHX_OK;
 }
 int orc_l2_error(void* h, const double* u, int32_t nm, const double* amp, const double* freq, double* l2,
                  double* l2g) {

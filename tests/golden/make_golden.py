@@ -86,6 +86,27 @@ NS_CASES = {
 }
 
 
+# transient: regression/thermal/2D_verification_transient (BWE, BDF order 1,
+# 40x40 Q1, dt = 1/20, T(t=0) = 0, T = 0 on the boundary); the source
+# (8 pi^2 sin(2 pi t) + 2 pi cos(2 pi t)) sin(2 pi x) sin(2 pi y) is stored as
+# the two time factors' amplitudes, the true solution sin(2 pi t) sin(2 pi x) sin(2 pi y)
+TRANSIENT_CASES = {
+    "2D_verification_transient": dict(dim=2, order=1, quadrature=2, n=[40, 40], workset=100, final_time=1.0,
+                                      steps=20, bdf_order=1, tableau="BWE",
+                                      source_sin_amp_pi2=8.0, source_cos_amp_pi=2.0, omega_pi=2.0,
+                                      space_freq_pi=[2, 2]),
+}
+
+
+def transient_norms(path: Path) -> list:
+    out = []
+    for line in path.read_text().splitlines():
+        m = re.search(r"L2 norm of the error for (\S+) = (\S+)\s+\(time = (\S+)\)", line)
+        if m:
+            out.append([float(m.group(3)), float(m.group(2))])
+    return out
+
+
 def main(ref: Path):
     kat = parse_hgrad(ref / "regression/discretization/HGRAD/mrhyde.gold")
     (HERE / "hgrad_kat.json").write_text(json.dumps(kat, indent=1))
@@ -102,7 +123,13 @@ def main(ref: Path):
     ns["_note"] = ("velocity Dirichlet sides as a mask (left 1, right 2, bottom 4, top 8); true solution "
                    "ux = 0.5*y*(1-y), uy = pr = 0; density and viscosity at their defaults (1)")
     (HERE / "ns_gold.json").write_text(json.dumps(ns, indent=1))
-    print("wrote", HERE / "hgrad_kat.json", HERE / "thermal_gold.json", HERE / "ns_gold.json")
+    tr = {}
+    for name, deck in TRANSIENT_CASES.items():
+        tr[name] = dict(deck, gold=transient_norms(ref / "regression/thermal" / name / "mrhyde.gold"))
+    tr["_note"] = "gold: [time, L2 norm of the error for T] per step"
+    (HERE / "transient_gold.json").write_text(json.dumps(tr, indent=1))
+    print("wrote", HERE / "hgrad_kat.json", HERE / "thermal_gold.json", HERE / "ns_gold.json",
+          HERE / "transient_gold.json")
 
 
 if __name__ == "__main__":
