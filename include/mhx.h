@@ -97,6 +97,24 @@ typedef struct mhx_problem_desc {
   double  nq_freq[MHX_MAX_NQ_MODES][3];
   double  nq_phase[MHX_MAX_NQ_MODES][3];
 
+  /* weak Dirichlet sides (thermal boundaryResidual "weak Dirichlet",
+   * src/physics/thermal.cpp:226-281, Nitsche): on the sides in wdbc_sides
+   *   res += int_side (10/h) k (T - g) phi - k (grad T . n) phi
+   *                   - s k (T - g) (grad phi . n),
+   * h = (side measure)^(1/(dim-1)) per element side, k the thermal
+   * diffusion at T, s the physics' "form_param" (wdbc_form 0: s = 1, the
+   * reference default; 1: s = -1; 2: s = 0), and the data
+   * g(x) = wd_const + wd_lin . x + sum_m wd_amp[m] prod_d sin(wd_freq[m][d] x_d + wd_phase[m][d]). */
+  uint32_t wdbc_sides;
+  int32_t  wdbc_form;
+  double  wd_const;
+  double  wd_lin[3];
+  int32_t n_wd_modes;     /* <= MHX_MAX_NQ_MODES */
+  int32_t wd_reserved;
+  double  wd_amp[MHX_MAX_NQ_MODES];
+  double  wd_freq[MHX_MAX_NQ_MODES][3];
+  double  wd_phase[MHX_MAX_NQ_MODES][3];
+
   /* strong Dirichlet data (reference: "Dirichlet conditions", use strong DBCs) */
   uint32_t dbc_sides;     /* MHX_SIDE_* mask */
   double  dbc_const;      /* value when dbc_values == NULL */

@@ -53,6 +53,10 @@ class ProblemDesc(C.Structure):
         ("neumann_sides", C.c_uint32), ("n_nq_modes", C.c_int32),
         ("nq_amp", (C.c_double * 3) * MHX_MAX_NQ_MODES), ("nq_freq", (C.c_double * 3) * MHX_MAX_NQ_MODES),
         ("nq_phase", (C.c_double * 3) * MHX_MAX_NQ_MODES),
+        ("wdbc_sides", C.c_uint32), ("wdbc_form", C.c_int32), ("wd_const", C.c_double),
+        ("wd_lin", C.c_double * 3), ("n_wd_modes", C.c_int32), ("wd_reserved", C.c_int32),
+        ("wd_amp", C.c_double * MHX_MAX_NQ_MODES), ("wd_freq", (C.c_double * 3) * MHX_MAX_NQ_MODES),
+        ("wd_phase", (C.c_double * 3) * MHX_MAX_NQ_MODES),
         ("dbc_sides", C.c_uint32), ("dbc_const", C.c_double), ("dbc_values", _dp),
         ("ns_rho", C.c_double), ("ns_mu", C.c_double), ("ns_supg", C.c_int32), ("ns_pspg", C.c_int32),
         ("ns_lid_sides", C.c_uint32), ("ns_lid_u", C.c_double * 3),
@@ -173,6 +177,13 @@ class Problem:
     # q_c = sum amp_c prod_d sin(freq_d x_d + phase_d); g = q . n
     neumann_sides: int = 0
     neumann_flux: Sequence = ()
+    # weak (Nitsche) Dirichlet sides: data g = const + lin . x + sum amp prod sin(freq x + phase),
+    # terms (amp, freq[3], phase[3]); form 0 symmetric (form_param 1), 1 non-symmetric (-1), 2 incomplete (0)
+    wdbc_sides: int = 0
+    wdbc_form: int = 0
+    wd_const: float = 0.0
+    wd_lin: Sequence[float] = (0.0, 0.0, 0.0)
+    wd_modes: Sequence = ()
     dbc_sides: int = MHX_SIDE_ALL
     dbc_const: float = 0.0
     dbc_values: Optional[np.ndarray] = None
@@ -250,6 +261,17 @@ class Problem:
                 d.nq_amp[m][a] = float((list(amp) + [0.0] * 3)[a])
                 d.nq_freq[m][a] = float((list(freq) + [0.0] * 3)[a])
                 d.nq_phase[m][a] = float((list(phase) + [0.0] * 3)[a])
+        if len(self.wd_modes) > MHX_MAX_NQ_MODES:
+            raise ValueError("too many weak Dirichlet data terms")
+        d.wdbc_sides, d.wdbc_form, d.wd_const = int(self.wdbc_sides), int(self.wdbc_form), float(self.wd_const)
+        d.n_wd_modes = len(self.wd_modes)
+        for a in range(3):
+            d.wd_lin[a] = float((list(self.wd_lin) + [0.0] * 3)[a])
+        for m, (amp, freq, phase) in enumerate(self.wd_modes):
+            d.wd_amp[m] = float(amp)
+            for a in range(3):
+                d.wd_freq[m][a] = float((list(freq) + [0.0] * 3)[a])
+                d.wd_phase[m][a] = float((list(phase) + [0.0] * 3)[a])
         d.dbc_sides, d.dbc_const = int(self.dbc_sides), float(self.dbc_const)
         d.dbc_values = arr(self.dbc_values, self.n_dof)
         d.ns_rho, d.ns_mu = float(self.rho), float(self.mu)

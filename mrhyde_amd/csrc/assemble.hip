@@ -35,6 +35,7 @@
 
 #include <cmath>
 
+#include "dual.hpp"
 #include "lattice.hpp"
 #include "mhx_internal.hpp"
 
@@ -148,52 +149,6 @@ struct Cfg {
   static constexpr int GPB = BLOCK / G;
   static_assert(G <= 64, "element needs more than one wavefront");
 };
-
-// ---------------------------------------------------------------------------
-// point-level forward-mode dual (value + N derivative slots)
-// ---------------------------------------------------------------------------
-template <int N>
-struct QDual {
-  double v;
-  double d[N];
-};
-template <int N>
-__device__ __forceinline__ QDual<N> qd_const(double x) {
-  QDual<N> r;
-  r.v = x;
-#pragma unroll
-  for (int i = 0; i < N; ++i) r.d[i] = 0.0;
-  return r;
-}
-template <int N>
-__device__ __forceinline__ QDual<N> qd_seed(double x, int slot) {
-  QDual<N> r = qd_const<N>(x);
-#pragma unroll
-  for (int i = 0; i < N; ++i) r.d[i] = (i == slot) ? 1.0 : 0.0;
-  return r;
-}
-template <int N>
-__device__ __forceinline__ QDual<N> operator*(const QDual<N>& a, const QDual<N>& b) {
-  QDual<N> r;
-  r.v = a.v * b.v;
-#pragma unroll
-  for (int i = 0; i < N; ++i) r.d[i] = a.d[i] * b.v + a.v * b.d[i];
-  return r;
-}
-template <int N>
-__device__ __forceinline__ QDual<N> operator*(double a, const QDual<N>& b) {
-  QDual<N> r;
-  r.v = a * b.v;
-#pragma unroll
-  for (int i = 0; i < N; ++i) r.d[i] = a * b.d[i];
-  return r;
-}
-template <int N>
-__device__ __forceinline__ QDual<N> operator+(double a, const QDual<N>& b) {
-  QDual<N> r = b;
-  r.v = a + b.v;
-  return r;
-}
 
 // ---------------------------------------------------------------------------
 // thermal physics at one point (thermal.cpp:125-163, steady, no advection):
@@ -1032,7 +987,9 @@ void assemble_thermal(Ctx& c, const double* u, double* res, double* jac, bool wa
   {
     ScopedPhase ph(c.prof, want_jac ? "AssemblyManager::assembleJacRes" : "AssemblyManager::assembleRes", c.stream);
     f(c, a, want_jac);
-    if (want_res) neumann_residual(c, res);  // boundary groups, after the volume groups
+    // boundary groups, after the volume groups
+    if (want_res) neumann_residual(c, res);
+    nitsche_boundary(c, u, want_res ? res : nullptr, want_jac ? jac : nullptr);
   }
   if (accumulate && want_jac && !c.h_dbc_list.empty()) {
     ScopedPhase ph(c.prof, "AssemblyManager::dofConstraints", c.stream);

@@ -72,6 +72,16 @@ static void validate(const mhx_problem_desc* d) {
   MHX_REQUIRE(d->n_nq_modes >= 0 && d->n_nq_modes <= MHX_MAX_NQ_MODES, MHX_ERR_ARG, "n_nq_modes out of range");
   MHX_REQUIRE(d->th_density >= 0.0 && d->th_specific_heat >= 0.0, MHX_ERR_ARG,
               "density and specific heat must be >= 0 (0 = 1.0)");
+  if (d->wdbc_sides) {
+    MHX_REQUIRE(d->physics == MHX_PHYS_THERMAL, MHX_ERR_UNSUPPORTED,
+                "weak Dirichlet sides are built for the thermal module only");
+    MHX_REQUIRE(nitsche_supported(d->dim, d->order), MHX_ERR_UNSUPPORTED,
+                "weak Dirichlet sides: no kernel built for this dimension and order");
+    MHX_REQUIRE((d->wdbc_sides & (d->dbc_sides | d->neumann_sides)) == 0, MHX_ERR_ARG,
+                "a side cannot be weak Dirichlet and strong Dirichlet or Neumann at once");
+    MHX_REQUIRE(d->wdbc_form >= 0 && d->wdbc_form <= 2, MHX_ERR_ARG, "wdbc_form must be 0, 1 or 2");
+    MHX_REQUIRE(d->n_wd_modes >= 0 && d->n_wd_modes <= MHX_MAX_NQ_MODES, MHX_ERR_ARG, "n_wd_modes out of range");
+  }
   MHX_REQUIRE(d->neumann_sides == 0 || d->physics == MHX_PHYS_THERMAL, MHX_ERR_UNSUPPORTED,
               "Neumann sides are built for the thermal module only");
   if (d->physics == MHX_PHYS_NAVIER_STOKES) {

@@ -232,6 +232,10 @@ void assemble_thermal(Ctx& c, const double* u, double* res, double* jac, bool wa
                       bool accumulate);
 // thermal Neumann boundary residual (bc.hip)
 void neumann_residual(Ctx& c, double* res);
+// thermal weak (Nitsche) Dirichlet sides: residual and Jacobian terms added
+// after the volume assembly (bc.hip)
+bool nitsche_supported(int dim, int p);
+void nitsche_boundary(Ctx& c, const double* u, double* res, double* jac);
 // navier stokes assembly (ns.hip)
 bool ns_supported(int dim, int p, int nq1);
 void assemble_ns(Ctx& c, const double* u, double* res, double* jac, bool want_res, bool want_jac, bool accumulate);

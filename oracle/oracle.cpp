@@ -718,12 +718,190 @@ static void assemble_ns(const Problem& P, const double* u, bool want_jac, double
       if (P.fixed[g]) jv[find_slot(P, (int)g, (int)g)] = 1.0;
 }
 
+// weak Dirichlet data g(x) = wd_const + wd_lin . x + sum_m amp prod sin(freq x + phase)
+static double weak_dbc_data(const mhx_problem_desc& d, int dim, const double* x) {
+  double g = d.wd_const;
+  for (int b = 0; b < dim; ++b) g += d.wd_lin[b] * x[b];
+  for (int m = 0; m < d.n_wd_modes; ++m) {
+    double prod = d.wd_amp[m];
+    for (int b = 0; b < dim; ++b) prod *= std::sin(d.wd_freq[m][b] * x[b] + d.wd_phase[m][b]);
+    g += prod;
+  }
+  return g;
+}
+
+// thermal<EvalT>::boundaryResidual for "weak Dirichlet" sides (thermal.cpp:
+// 226-281), on the boundary groups after the volume groups
+// (assemblyManager_jacres.hpp:389-401):
+//   res(e,dof) += epen/h k (T - g) w phi - k (grad T . n) w phi - sf k (T - g) w (grad phi . n)
+// with epen = 10, h = (sum of the side weights)^(1/(dim-1))
+// (Workset::getSideElementSize, workset.cpp:2719-2733), sf = "form_param"
+// (thermal.cpp:35), k the thermal diffusion at T.  Side cubature and normals
+// as in neumann_residual; grad phi through the element's Q1 map at the side
+// point.  Scatter as the volume: res_glob -= res, J += d res / d u, strong
+// Dirichlet rows skipped.
+template <class EvalT>
+static void nitsche_boundary(const Problem& P, const double* u, bool want_jac, double* res, double* jv) {
+  const mhx_problem_desc& d = P.desc;
+  if (!d.wdbc_sides) return;
+  const int dim = P.dim, p = P.p, nq1 = P.nq1, nb = P.nb;
+  const double epen = 10.0;
+  const double sf = d.wdbc_form == 1 ? -1.0 : d.wdbc_form == 2 ? 0.0 : 1.0;
+  std::vector<double> L(p + 1), dL(p + 1);
+  for (int s = 0; s < 2 * dim; ++s) {
+    if (!(d.wdbc_sides & (1u << s))) continue;
+    const int a = s / 2;
+    const bool hi = s & 1;
+    const int ta[2] = {a == 0 ? 1 : 0, a == 2 ? 1 : 2};
+    const int nt = dim - 1;
+    const int nqs = nt == 1 ? nq1 : nq1 * nq1;
+    for (long e = 0; e < P.ne; ++e) {
+      long ec[3];
+      P.elem_coords(e, ec);
+      if (ec[a] != (hi ? P.nel[a] - 1 : 0)) continue;
+      std::vector<double> X(P.nv * 3);
+      for (int v = 0; v < P.nv; ++v) P.vertex_of_elem(e, v, &X[v * 3]);
+      std::vector<EvalT> uAD(nb);
+      for (int j = 0; j < nb; ++j) {
+        const double uv = u[P.lids[e * nb + j]];
+        uAD[j] = want_jac ? Seed<EvalT>::make(j, uv) : EvalT(uv);
+      }
+      // side points: weights, normals, physical coordinates, basis and gradients
+      std::vector<double> wq(nqs), nq(nqs * 3), xq(nqs * 3), phi(nb * nqs), gphi(nb * nqs * dim);
+      for (int qs = 0; qs < nqs; ++qs) {
+        double xi[3] = {0, 0, 0}, w = 1.0;
+        xi[a] = hi ? 1.0 : -1.0;
+        const int qq[2] = {qs % nq1, qs / nq1};
+        for (int t = 0; t < nt; ++t) {
+          xi[ta[t]] = P.qx1[qq[t]];
+          w *= P.qw1[qq[t]];
+        }
+        double x[3] = {0, 0, 0}, tg[2][3] = {{0, 0, 0}, {0, 0, 0}}, J[3][3] = {{0}};
+        for (int v = 0; v < P.nv; ++v) {
+          double N = 1.0, dN[3] = {1.0, 1.0, 1.0};
+          for (int b = 0; b < dim; ++b) {
+            const double sgn = ((v >> b) & 1) ? 1.0 : -1.0, f = 0.5 * (1.0 + sgn * xi[b]);
+            N *= f;
+            for (int c = 0; c < dim; ++c) dN[c] *= (c == b) ? 0.5 * sgn : f;
+          }
+          for (int c = 0; c < 3; ++c) {
+            x[c] += N * X[v * 3 + c];
+            for (int t = 0; t < nt; ++t) tg[t][c] += dN[ta[t]] * X[v * 3 + c];
+          }
+          for (int r = 0; r < dim; ++r)
+            for (int b = 0; b < dim; ++b) J[r][b] += X[v * 3 + r] * dN[b];
+        }
+        double nrm[3] = {0, 0, 0}, area;
+        if (dim == 2) {
+          area = std::sqrt(tg[0][0] * tg[0][0] + tg[0][1] * tg[0][1]);
+          nrm[0] = tg[0][1] / area;
+          nrm[1] = -tg[0][0] / area;
+        } else {
+          nrm[0] = tg[0][1] * tg[1][2] - tg[0][2] * tg[1][1];
+          nrm[1] = tg[0][2] * tg[1][0] - tg[0][0] * tg[1][2];
+          nrm[2] = tg[0][0] * tg[1][1] - tg[0][1] * tg[1][0];
+          area = std::sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2]);
+          for (int c = 0; c < 3; ++c) nrm[c] /= area;
+        }
+        if (nrm[a] * (hi ? 1.0 : -1.0) < 0) for (int c = 0; c < 3; ++c) nrm[c] = -nrm[c];
+        wq[qs] = w * area;
+        for (int c = 0; c < 3; ++c) {
+          nq[qs * 3 + c] = nrm[c];
+          xq[qs * 3 + c] = x[c];
+        }
+        double Ji[3][3] = {{0}};
+        if (dim == 2) {
+          const double det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+          Ji[0][0] = J[1][1] / det; Ji[0][1] = -J[0][1] / det;
+          Ji[1][0] = -J[1][0] / det; Ji[1][1] = J[0][0] / det;
+        } else {
+          const double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+                             J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                             J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+          Ji[0][0] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]) / det;
+          Ji[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) / det;
+          Ji[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) / det;
+          Ji[1][0] = (J[1][2] * J[2][0] - J[1][0] * J[2][2]) / det;
+          Ji[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) / det;
+          Ji[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) / det;
+          Ji[2][0] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) / det;
+          Ji[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) / det;
+          Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) / det;
+        }
+        std::vector<double> Lb(dim * (p + 1)), Db(dim * (p + 1));
+        for (int b = 0; b < dim; ++b) {
+          lagrange1d(p, xi[b], L.data(), dL.data());
+          for (int k = 0; k <= p; ++k) {
+            Lb[b * (p + 1) + k] = L[k];
+            Db[b * (p + 1) + k] = dL[k];
+          }
+        }
+        for (int j = 0; j < nb; ++j) {
+          const int jl[3] = {j % (p + 1), (j / (p + 1)) % (p + 1), dim == 3 ? j / ((p + 1) * (p + 1)) : 0};
+          double ph = 1.0, gr[3];
+          for (int b = 0; b < dim; ++b) ph *= Lb[b * (p + 1) + jl[b]];
+          for (int c = 0; c < dim; ++c) {
+            gr[c] = 1.0;
+            for (int b = 0; b < dim; ++b) gr[c] *= (b == c ? Db : Lb)[b * (p + 1) + jl[b]];
+          }
+          phi[j * nqs + qs] = ph;
+          for (int c = 0; c < dim; ++c) {
+            double sg = 0.0;
+            for (int b = 0; b < dim; ++b) sg += Ji[b][c] * gr[b];
+            gphi[(j * nqs + qs) * dim + c] = sg;
+          }
+        }
+      }
+      double vol = 0.0;
+      for (int qs = 0; qs < nqs; ++qs) vol += wq[qs];
+      const double h = std::pow(vol, 1.0 / ((double)dim - 1.0));
+      std::vector<EvalT> r(nb, EvalT(0.0));
+      for (int qs = 0; qs < nqs; ++qs) {
+        // evaluateSolutionField on the side: field = u_0 b_0, then += u_dof b_dof
+        EvalT T = uAD[0] * phi[qs];
+        EvalT dT[3];
+        for (int c = 0; c < dim; ++c) dT[c] = uAD[0] * gphi[qs * dim + c];
+        for (int j = 1; j < nb; ++j) {
+          T += uAD[j] * phi[j * nqs + qs];
+          for (int c = 0; c < dim; ++c) dT[c] += uAD[j] * gphi[(j * nqs + qs) * dim + c];
+        }
+        EvalT TT = T * T;
+        EvalT onep = 1.0 + TT * d.diff_u2;
+        EvalT k = P.k0[e] * (d.diff_u2 != 0.0 ? onep : EvalT(1.0));
+        const double g = weak_dbc_data(d, dim, &xq[qs * 3]);
+        EvalT dTn = dT[0] * nq[qs * 3];
+        for (int c = 1; c < dim; ++c) dTn += dT[c] * nq[qs * 3 + c];
+        const double w = wq[qs];
+        for (int i = 0; i < nb; ++i) {
+          double gpn = 0.0;
+          for (int c = 0; c < dim; ++c) gpn += gphi[(i * nqs + qs) * dim + c] * nq[qs * 3 + c];
+          r[i] += epen / h * k * (T - g) * w * phi[i * nqs + qs];
+          r[i] += -k * dTn * w * phi[i * nqs + qs];
+          r[i] += -sf * k * (T - g) * w * gpn;
+        }
+      }
+      const int* Lr = &P.lids[e * nb];
+      for (int i = 0; i < nb; ++i) {
+        const int row = Lr[i];
+        if (P.fixed[row]) continue;
+        if (res) res[row] += -val(r[i]);
+        if (want_jac && jv)
+          for (int j = 0; j < nb; ++j) jv[find_slot(P, row, Lr[j])] += dx(r[i], j);
+      }
+    }
+  }
+}
+
 static void assemble(const Problem& P, const double* u, bool want_jac, double* res, double* jv) {
   const bool ns = P.desc.physics == MHX_PHYS_NAVIER_STOKES;
   auto run = [&](auto tag) {
     using EvalT = typename decltype(tag)::type;
-    if (ns) assemble_ns<EvalT>(P, u, want_jac, res, jv);
-    else assemble_thermal<EvalT>(P, u, want_jac, res, jv);
+    if (ns) {
+      assemble_ns<EvalT>(P, u, want_jac, res, jv);
+    } else {
+      assemble_thermal<EvalT>(P, u, want_jac, res, jv);
+      nitsche_boundary<EvalT>(P, u, want_jac, res, jv);  // boundary groups, after the volume groups
+    }
   };
   struct D { using type = double; };
   if (!want_jac) {
@@ -1051,6 +1229,16 @@ int orc_update(void* h, const mhx_problem_desc* d) {
     for (int a = 0; a < 3; ++a) P->desc.mode_freq[m][a] = d->mode_freq[m][a];
   P->desc.th_density = d->th_density;
   P->desc.th_specific_heat = d->th_specific_heat;
+  P->desc.wd_const = d->wd_const;
+  P->desc.n_wd_modes = d->n_wd_modes;
+  for (int b = 0; b < 3; ++b) P->desc.wd_lin[b] = d->wd_lin[b];
+  for (int m = 0; m < MHX_MAX_NQ_MODES; ++m) {
+    P->desc.wd_amp[m] = d->wd_amp[m];
+    for (int b = 0; b < 3; ++b) {
+      P->desc.wd_freq[m][b] = d->wd_freq[m][b];
+      P->desc.wd_phase[m][b] = d->wd_phase[m][b];
+    }
+  }
   if (!d->dbc_values) {
     P->desc.dbc_const = d->dbc_const;
     for (long g = 0; g < P->ndof; ++g)
