@@ -234,6 +234,21 @@ int  mhx_spmv(mhx_ctx* ctx, const double* jac_vals, const double* x, double* y, 
 int  mhx_linear_solve(mhx_ctx* ctx, const double* jac_vals, const double* r, double* x,
                       const mhx_krylov_opts* opts, mhx_solve_stats* stats);
 
+/* MatrixMarket files.  Replaces LinearAlgebraInterface::writeToFile
+ * (src/interfaces/linear_algebra/linearAlgebraInterface_util.hpp:316-331,
+ * Tpetra::MatrixMarket::Writer::writeSparseFile / writeDenseFile): J as
+ * "coordinate real general" with 1-based DOF indices in the engine's
+ * numbering, vectors as "array real general" (n x 1), values with 17
+ * significant digits (a read returns the same bits).  jac_vals == NULL: the
+ * context's Jacobian.  The readers place a file's entries into the context's
+ * CRS pattern (coordinate general or symmetric; duplicates summed); an entry
+ * outside the pattern is MHX_ERR_ARG.  MHX_DEVICE_PTRS in flags: the value
+ * pointers are device pointers. */
+int  mhx_write_matrix_market(mhx_ctx* ctx, const char* path, const double* jac_vals, int32_t flags);
+int  mhx_write_vector_market(mhx_ctx* ctx, const char* path, const double* vec, int32_t flags);
+int  mhx_read_matrix_market(mhx_ctx* ctx, const char* path, double* jac_vals, int32_t flags);
+int  mhx_read_vector_market(mhx_ctx* ctx, const char* path, double* vec, int32_t flags);
+
 /* Build the smoothed-aggregation hierarchy of J (jac_vals == NULL: the
  * context's Jacobian; MHX_DEVICE_PTRS in flags: device pointer) and keep it
  * in the context; a solve with MHX_PREC_AMG rebuilds it from its own J.

@@ -63,6 +63,10 @@ def _load() -> C.CDLL:
         "mhx_server_destroy": (None, [vp]),
         "mhx_amg_setup": (C.c_int, [vp, vp, i32]),
         "mhx_set_time_state": (C.c_int, [vp, C.c_double, i32, vp, i32]),
+        "mhx_write_matrix_market": (C.c_int, [vp, C.c_char_p, vp, i32]),
+        "mhx_write_vector_market": (C.c_int, [vp, C.c_char_p, vp, i32]),
+        "mhx_read_matrix_market": (C.c_int, [vp, C.c_char_p, vp, i32]),
+        "mhx_read_vector_market": (C.c_int, [vp, C.c_char_p, vp, i32]),
         "mhx_amg_level": (C.c_int, [vp, i32, i32, C.POINTER(i64), C.POINTER(C.c_double), vp, vp, vp]),
     }
     for name, (res, args) in sig.items():
@@ -83,5 +87,6 @@ EXPORTED = [
     "mhx_batch_results", "mhx_batch_info", "mhx_batch_set_wg", "mhx_batch_destroy","mhx_state_ptr", "mhx_state_upload", "mhx_state_download", "mhx_rng_fill", "mhx_version",
     "mhx_footprint", "mhx_ctx_update", "mhx_server_create", "mhx_server_submit", "mhx_server_poll",
     "mhx_server_info", "mhx_server_last_error", "mhx_server_destroy", "mhx_amg_setup", "mhx_amg_level",
-    "mhx_set_time_state",
+    "mhx_set_time_state", "mhx_write_matrix_market", "mhx_write_vector_market", "mhx_read_matrix_market",
+    "mhx_read_vector_market",
 ]

@@ -560,6 +560,44 @@ int mhx_set_time_state(mhx_ctx* ctx, double dt, int32_t bdf_order, const double*
   MHX_GUARD_END(ctx)
 }
 
+int mhx_write_matrix_market(mhx_ctx* ctx, const char* path, const double* jac_vals, int32_t flags) {
+  MHX_GUARD_BEGIN
+  MHX_REQUIRE(ctx && path, MHX_ERR_ARG, "null argument");
+  bool dev = (flags & MHX_DEVICE_PTRS) != 0;
+  if (!jac_vals) {
+    MHX_REQUIRE(ctx->jac_valid && ctx->jac.p, MHX_ERR_STATE, "no Jacobian held by the context");
+    jac_vals = ctx->jac.p;
+    dev = true;
+  }
+  write_matrix_market(*ctx, path, jac_vals, dev);
+  return MHX_OK;
+  MHX_GUARD_END(ctx)
+}
+
+int mhx_write_vector_market(mhx_ctx* ctx, const char* path, const double* vec, int32_t flags) {
+  MHX_GUARD_BEGIN
+  MHX_REQUIRE(ctx && path && vec, MHX_ERR_ARG, "null argument");
+  write_vector_market(*ctx, path, vec, (flags & MHX_DEVICE_PTRS) != 0);
+  return MHX_OK;
+  MHX_GUARD_END(ctx)
+}
+
+int mhx_read_matrix_market(mhx_ctx* ctx, const char* path, double* jac_vals, int32_t flags) {
+  MHX_GUARD_BEGIN
+  MHX_REQUIRE(ctx && path && jac_vals, MHX_ERR_ARG, "null argument");
+  read_matrix_market(*ctx, path, jac_vals, (flags & MHX_DEVICE_PTRS) != 0);
+  return MHX_OK;
+  MHX_GUARD_END(ctx)
+}
+
+int mhx_read_vector_market(mhx_ctx* ctx, const char* path, double* vec, int32_t flags) {
+  MHX_GUARD_BEGIN
+  MHX_REQUIRE(ctx && path && vec, MHX_ERR_ARG, "null argument");
+  read_vector_market(*ctx, path, vec, (flags & MHX_DEVICE_PTRS) != 0);
+  return MHX_OK;
+  MHX_GUARD_END(ctx)
+}
+
 int mhx_amg_setup(mhx_ctx* ctx, const double* jac_vals, int32_t flags) {
   MHX_GUARD_BEGIN
   MHX_REQUIRE(ctx, MHX_ERR_ARG, "null argument");

@@ -250,6 +250,12 @@ int krylov_solve(Ctx& c, const double* A, const double* b, double* x, const mhx_
 void set_dirichlet_dev(Ctx& c, double* u);
 void axpy(Ctx& c, long n, double a, const double* x, double* y);
 
+// MatrixMarket files (mmio.hip); dev: the pointer is a device pointer
+void write_matrix_market(Ctx& c, const char* path, const double* vals, bool dev);
+void write_vector_market(Ctx& c, const char* path, const double* vec, bool dev);
+void read_matrix_market(Ctx& c, const char* path, double* vals, bool dev);
+void read_vector_market(Ctx& c, const char* path, double* vec, bool dev);
+
 // smoothed-aggregation AMG (amg.hip): set-up from the FP64 operator A on the
 // context's pattern (and FP32 copies when fp32), and one V-cycle out = M^-1 v
 // with level-0 operator A (the FP64 A, or its FP32 copy)

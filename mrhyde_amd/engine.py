@@ -298,6 +298,32 @@ class Mhx:
             self._check(rc, "nonlinearSolver")
         return u, st.as_dict()
 
+    def writeToFile(self, jac=None, res=None, sol=None, jac_filename: str = "jacobian.mm",
+                    res_filename: str = "residual.mm", sol_filename: str = "solution.mm"):
+        """LinearAlgebraInterface::writeToFile: the given J (values in the
+        context's pattern), residual and solution as MatrixMarket files;
+        None skips a file (for J: pass "context" for the Newton Jacobian)."""
+        if jac is not None:
+            pj, dj = (None, False) if isinstance(jac, str) else _ptr(jac)
+            self._check(lib.mhx_write_matrix_market(self._h, jac_filename.encode(), pj,
+                                                    capi.MHX_DEVICE_PTRS if dj else 0), "write_matrix_market")
+        for vec, name in ((res, res_filename), (sol, sol_filename)):
+            if vec is not None:
+                pv, dv = _ptr(vec)
+                self._check(lib.mhx_write_vector_market(self._h, name.encode(), pv,
+                                                        capi.MHX_DEVICE_PTRS if dv else 0), "write_vector_market")
+
+    def readMatrixMarket(self, path: str) -> np.ndarray:
+        """Values of a MatrixMarket matrix in the context's CRS pattern."""
+        out = np.zeros(self.nnz)
+        self._check(lib.mhx_read_matrix_market(self._h, path.encode(), out.ctypes.data, 0), "read_matrix_market")
+        return out
+
+    def readVectorMarket(self, path: str) -> np.ndarray:
+        out = np.zeros(self.n_dof)
+        self._check(lib.mhx_read_vector_market(self._h, path.encode(), out.ctypes.data, 0), "read_vector_market")
+        return out
+
     def update(self, problem: Problem):
         """New coefficients on the same mesh (mhx_ctx_update): k0, sources,
         density / specific heat, Dirichlet values."""
