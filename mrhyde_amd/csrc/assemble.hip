@@ -804,14 +804,15 @@ __global__ __launch_bounds__(256) void k_thermal(AsmArgs a) {
       // folds at compile time, and for a fixed row the group's lanes write the
       // row's runs of consecutive slots.  Per chunk of rows: slots and flags
       // first, the loads that are needed (later touches of non-Dirichlet
-      // rows) issued together, then the stores.
+      // rows) issued together, then the stores; the next chunk's loads are
+      // issued before this chunk's stores (double buffer).
       const int jl[3] = {lane % P1, (lane / P1) % P1, DIM == 3 ? lane / (P1 * P1) : 0};
       constexpr int CH = NB < 9 ? NB : 9;
-#pragma unroll
-      for (int i0 = 0; i0 < NB; i0 += CH) {
-        int sl[CH];
-        uint32_t first_mask = 0, fixed_mask = 0;
-        double old[CH];
+      constexpr int NCH = (NB + CH - 1) / CH;
+      // slots and flags of chunk c, and the loads of the entries it must add to
+      auto prep = [&](int i0, int* sl, uint32_t& first_mask, uint32_t& fixed_mask, double* old) {
+        first_mask = 0;
+        fixed_mask = 0;
 #pragma unroll
         for (int k = 0; k < CH; ++k) {
           const int i = i0 + k;
@@ -830,9 +831,25 @@ __global__ __launch_bounds__(256) void k_thermal(AsmArgs a) {
           if (fx) fixed_mask |= 1u << k;
           old[k] = (!first && !fx) ? a.jac[sl[k]] : 0.0;
         }
+      };
+      // double buffer: the loads of chunk c+1 are issued before the stores of
+      // chunk c, so their latency overlaps
+      int slA[CH], slB[CH];
+      uint32_t fmA, xmA, fmB, xmB;
+      double oldA[CH], oldB[CH];
+      prep(0, slA, fmA, xmA, oldA);
+#pragma unroll
+      for (int c = 0; c < NCH; ++c) {
+        int* sl = (c & 1) ? slB : slA;
+        double* old = (c & 1) ? oldB : oldA;
+        const uint32_t first_mask = (c & 1) ? fmB : fmA, fixed_mask = (c & 1) ? xmB : xmA;
+        if (c + 1 < NCH) {
+          if (c & 1) prep((c + 1) * CH, slA, fmA, xmA, oldA);
+          else prep((c + 1) * CH, slB, fmB, xmB, oldB);
+        }
 #pragma unroll
         for (int k = 0; k < CH; ++k) {
-          const int i = i0 + k;
+          const int i = c * CH + k;
           if (i >= NB) break;
           const bool first = (first_mask >> k) & 1u, fx = (fixed_mask >> k) & 1u;
           // streaming stores: a later colour re-reads an entry long after
