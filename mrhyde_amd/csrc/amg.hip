@@ -51,6 +51,13 @@ constexpr double kAmgDamping = 4.0 / 3.0;
 constexpr double kChebRatio = 7.0;     // "chebyshev: ratio eigenvalue"
 constexpr double kChebBoost = 1.1;     // Ifpack2 Chebyshev boost of the estimate
 constexpr int kAmgMaxFields = 4;
+// smoothers: Chebyshev(2) on D^-1 A (the reference's MueLu setting), or two
+// damped sweeps of nodal block Jacobi (the nf x nf block of each node
+// inverted), for coupled multi-field systems whose point diagonal does not
+// bound the spectrum (stabilised Navier-Stokes)
+constexpr int kSmootherChebyshev = 0, kSmootherBlockJacobi = 1, kSmootherBlockGaussSeidel = 2;
+constexpr double kBjOmega = 0.6;
+constexpr int kBjSweeps = 2;
 constexpr int kAmgBlk = 128;           // threads per workgroup of the set-up kernels
 
 // ---------------------------------------------------------------------------
@@ -533,6 +540,84 @@ __global__ void k_amg_cast(long n, const TI* __restrict__ a, TO* __restrict__ b)
   if (i < n) b[i] = (TO)a[i];
 }
 
+// nodal block inverses: the BS x BS diagonal block of node k (DOFs k*BS ..
+// k*BS+BS-1, consecutive columns in every sorted row), Gauss-Jordan with
+// partial pivoting in FP64; a zero pivot leaves a zero block (no smoothing of
+// that node) instead of Inf
+template <int BS>
+__global__ void k_amg_block_inv(long nblk, const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
+                                const double* __restrict__ A, double* __restrict__ out) {
+  const long blk = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (blk >= nblk) return;
+  double M[BS][BS], I[BS][BS];
+  const int c0 = (int)(blk * BS);
+  for (int i = 0; i < BS; ++i) {
+    int lo = rp[c0 + i], hi = rp[c0 + i + 1];
+    const int end = hi;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (ci[mid] < c0) lo = mid + 1;
+      else hi = mid;
+    }
+    int k = lo;
+    for (int j = 0; j < BS; ++j) {
+      const bool hit = k < end && ci[k] == c0 + j;
+      M[i][j] = hit ? A[k] : 0.0;
+      k += hit;
+      I[i][j] = i == j ? 1.0 : 0.0;
+    }
+  }
+  bool ok = true;
+  for (int col = 0; col < BS; ++col) {
+    int piv = col;
+    double best = fabs(M[col][col]);
+    for (int r = col + 1; r < BS; ++r)
+      if (fabs(M[r][col]) > best) {
+        best = fabs(M[r][col]);
+        piv = r;
+      }
+    if (!(best > 0.0)) {
+      ok = false;
+      break;
+    }
+    if (piv != col)
+      for (int j = 0; j < BS; ++j) {
+        double t = M[col][j]; M[col][j] = M[piv][j]; M[piv][j] = t;
+        t = I[col][j]; I[col][j] = I[piv][j]; I[piv][j] = t;
+      }
+    const double d = 1.0 / M[col][col];
+    for (int j = 0; j < BS; ++j) {
+      M[col][j] *= d;
+      I[col][j] *= d;
+    }
+    for (int r = 0; r < BS; ++r) {
+      if (r == col) continue;
+      const double f = M[r][col];
+      for (int j = 0; j < BS; ++j) {
+        M[r][j] = fma(-f, M[col][j], M[r][j]);
+        I[r][j] = fma(-f, I[col][j], I[r][j]);
+      }
+    }
+  }
+  for (int i = 0; i < BS; ++i)
+    for (int j = 0; j < BS; ++j) out[blk * BS * BS + i * BS + j] = ok ? I[i][j] : 0.0;
+}
+
+// xout = (zero_x ? 0 : xin) + omega Binv r, one DOF per thread (in place allowed)
+template <class T>
+__global__ void k_amg_bj(long n, int nf, const T* __restrict__ binv, const T* __restrict__ r, const T* xin, T* xout,
+                         double omega, int zero_x) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const long node = i / nf;
+  const int f = (int)(i - node * nf);
+  const T* row = binv + node * nf * nf + f * nf;
+  const T* rb = r + node * nf;
+  double s = 0.0;
+  for (int g = 0; g < nf; ++g) s = fma((double)row[g], (double)rb[g], s);
+  xout[i] = (T)((zero_x ? 0.0 : (double)xin[i]) + omega * s);
+}
+
 // ---------------------------------------------------------------------------
 // V-cycle kernels (T = double or float; sums in FP64)
 // ---------------------------------------------------------------------------
@@ -591,6 +676,49 @@ __global__ __launch_bounds__(256) void k_amg_resid(long n, const int32_t* __rest
   if (l == 0) r[row] = (T)((double)b[row] - ax);
 }
 
+// One colour of a multicolour nodal block Gauss-Seidel sweep: for every node
+// of the colour, x_node += Binv_node (b - A x)_node.  Colour (cx, cy, cz) of
+// stride s holds the nodes whose lattice coordinates are congruent to it
+// modulo s; with s = reach + 1 no two nodes of a colour share a matrix entry,
+// so the in-place update is race free.  A group of NFP * LANES lanes (NFP = nf
+// rounded up to a power of two, groups never straddle a wavefront) owns a
+// node, LANES lanes per row; the rows' residuals meet through wavefront
+// shuffles, which every lane executes.
+template <int LANES, int NFP, class T>
+__global__ __launch_bounds__(256) void k_amg_gs(long ncol, int nf, int s, int cx, int cy, int cz, long cntx, long cnty,
+                                                long Nx, long Ny, const int32_t* __restrict__ rp,
+                                                const int32_t* __restrict__ ci, const T* __restrict__ A,
+                                                const T* __restrict__ binv, const T* __restrict__ b, T* x) {
+  static_assert(NFP * LANES <= 64 && 64 % (NFP * LANES) == 0, "node group must tile a wavefront");
+  const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long t = gid / (NFP * LANES);
+  const int within = (int)(gid % (NFP * LANES));
+  const int f = within / LANES, l = within % LANES;
+  const bool active = t < ncol && f < nf;
+  long node = 0, row = 0;
+  double res = 0.0;
+  if (active) {
+    const long a = t % cntx, q = t / cntx;
+    node = (cx + s * a) + Nx * ((cy + s * (q % cnty)) + Ny * (cz + s * (q / cnty)));
+    row = node * nf + f;
+    res = (double)b[row] - row_dot<LANES, T>(rp, ci, A, x, row, l);
+  }
+  const int base = (int)(threadIdx.x & 63) - within;
+  double upd = 0.0;
+#pragma unroll
+  for (int g = 0; g < NFP; ++g) {
+    const double rg = __shfl(res, base + g * LANES, 64);
+    if (active && g < nf) upd = fma((double)binv[(node * nf + f) * nf + g], rg, upd);
+  }
+  if (active && l == 0) x[row] = (T)((double)x[row] + upd);
+}
+
+template <class T>
+__global__ void k_amg_fill0(long n, T* x) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = (T)0;
+}
+
 // y = M x (add = 0) or y += M x (add = 1)
 template <int LANES, class T>
 __global__ __launch_bounds__(256) void k_amg_spmv(long n, const int32_t* __restrict__ rp,
@@ -624,7 +752,7 @@ __global__ __launch_bounds__(256) void k_amg_gemv(long n, const double* __restri
 // ---------------------------------------------------------------------------
 template <class T>
 struct AmgVals {
-  DevBuf<T> A, dinv, P, R, cinv;
+  DevBuf<T> A, dinv, P, R, cinv, binv;
   DevBuf<T> x, t, b, r, d;  // level work vectors
 };
 
@@ -639,6 +767,7 @@ struct AmgLevel {
   DevBuf<double> A, dinv, ns;
   DevBuf<int32_t> prp, pci, rrp, rci, tab;
   DevBuf<double> P, R;
+  DevBuf<double> binv;  // nodal nf x nf diagonal-block inverses (block-Jacobi smoother)
   double lambda = 1.0;
   AmgVals<float> f32;
   AmgVals<double> f64;  // FP64 work vectors (values live in A/P/R above)
@@ -651,6 +780,8 @@ struct Amg {
   DevBuf<int> err;
   const double* A0 = nullptr;  // level-0 operator of the last set-up
   bool fp32 = false;
+  int smoother = kSmootherChebyshev;
+  int nf = 1;
   int64_t setup_count = 0;
 };
 
@@ -736,6 +867,8 @@ void amg_setup(Ctx& c, const double* A, bool fp32) {
   MHX_HIP(hipMemsetAsync(H.err.p, 0, sizeof(int), c.stream));
   H.A0 = A;
   H.fp32 = fp32;
+  H.smoother = c.amg_smoother;
+  H.nf = nf;
   ++H.setup_count;
 
   // level 0: the context's lattice and pattern
@@ -839,6 +972,21 @@ void amg_setup(Ctx& c, const double* A, bool fp32) {
     MHX_HIP(hipGetLastError());
     check_err(c, H, "coarsest");
   }
+  // block-Jacobi smoother: nodal block inverses of every smoothed level
+  if (H.smoother != kSmootherChebyshev) {
+    for (int l = 0; l + 1 < nlev; ++l) {
+      AmgLevel& V = *H.lv[l];
+      const long nblk = V.n / nf;
+      V.binv.alloc((size_t)nblk * nf * nf);
+      const double* Al = l ? V.A.p : A;
+      switch (nf) {
+        case 1: hipLaunchKernelGGL(k_amg_block_inv<1>, g1(nblk), dim3(256), 0, c.stream, nblk, V.rp, V.ci, Al, V.binv.p); break;
+        case 2: hipLaunchKernelGGL(k_amg_block_inv<2>, g1(nblk), dim3(256), 0, c.stream, nblk, V.rp, V.ci, Al, V.binv.p); break;
+        case 3: hipLaunchKernelGGL(k_amg_block_inv<3>, g1(nblk), dim3(256), 0, c.stream, nblk, V.rp, V.ci, Al, V.binv.p); break;
+        default: hipLaunchKernelGGL(k_amg_block_inv<4>, g1(nblk), dim3(256), 0, c.stream, nblk, V.rp, V.ci, Al, V.binv.p); break;
+      }
+    }
+  }
   // FP32 copies of the operators
   for (int l = 0; l < nlev; ++l) {
     AmgLevel& V = *H.lv[l];
@@ -853,6 +1001,7 @@ void amg_setup(Ctx& c, const double* A, bool fp32) {
         hipLaunchKernelGGL((k_amg_cast<double, float>), g1(m), dim3(256), 0, c.stream, m, s, d.p);
       };
       cast(V.dinv.p, V.f32.dinv, V.n);
+      if (H.smoother != kSmootherChebyshev && !last) cast(V.binv.p, V.f32.binv, V.binv.n);
       if (l > 0) cast(V.A.p, V.f32.A, V.A.n);
       if (!last) {
         cast(V.P.p, V.f32.P, V.P.n);
@@ -878,7 +1027,7 @@ static Cheb cheb_coef(double lambda) {
 
 template <class T>
 struct LvView {
-  const T *A, *dinv, *P, *R;
+  const T *A, *dinv, *P, *R, *binv;
   T *x, *t, *b, *r, *d;
 };
 
@@ -887,13 +1036,13 @@ static LvView<T> view(Amg& H, int l, const T* A0);
 template <>
 LvView<double> view<double>(Amg& H, int l, const double* A0) {
   AmgLevel& V = *H.lv[l];
-  return {l ? V.A.p : A0, V.dinv.p, V.P.p, V.R.p, V.f64.x.p, V.f64.t.p, V.f64.b.p, V.f64.r.p, V.f64.d.p};
+  return {l ? V.A.p : A0, V.dinv.p, V.P.p, V.R.p, V.binv.p, V.f64.x.p, V.f64.t.p, V.f64.b.p, V.f64.r.p, V.f64.d.p};
 }
 template <>
 LvView<float> view<float>(Amg& H, int l, const float* A0) {
   AmgLevel& V = *H.lv[l];
-  return {l ? V.f32.A.p : A0, V.f32.dinv.p, V.f32.P.p, V.f32.R.p, V.f32.x.p, V.f32.t.p, V.f32.b.p, V.f32.r.p,
-          V.f32.d.p};
+  return {l ? V.f32.A.p : A0, V.f32.dinv.p, V.f32.P.p, V.f32.R.p, V.f32.binv.p, V.f32.x.p, V.f32.t.p, V.f32.b.p,
+          V.f32.r.p, V.f32.d.p};
 }
 
 template <class T>
@@ -906,6 +1055,58 @@ static void cheb_step(Ctx& c, AmgLevel& V, const LvView<T>& W, const T* b, const
   });
 }
 
+// damped nodal block-Jacobi sweeps: x <- x + omega Binv (b - A x); with
+// zero_x the first sweep starts from x = 0 and needs no residual
+template <class T>
+static void bj_smooth(Ctx& c, Amg& H, AmgLevel& V, const LvView<T>& W, const T* b, T* x, int zero_x) {
+  for (int s = 0; s < kBjSweeps; ++s) {
+    const T* r = b;
+    if (!(zero_x && s == 0)) {
+      by_lanes(V.lanes, [&](auto LN) {
+        constexpr int K = decltype(LN)::value;
+        hipLaunchKernelGGL((k_amg_resid<K, T>), g1(V.n * K), dim3(256), 0, c.stream, V.n, V.rp, V.ci, W.A, b, x, W.t);
+      });
+      r = W.t;
+    }
+    hipLaunchKernelGGL(k_amg_bj<T>, g1(V.n), dim3(256), 0, c.stream, V.n, H.nf, W.binv, r, x, x, kBjOmega,
+                       (int)(zero_x && s == 0));
+  }
+}
+
+// one multicolour block Gauss-Seidel sweep over the (reach+1)^d colours of
+// the level's node lattice, in lexicographic colour order or its reverse
+template <class T>
+static void gs_sweep(Ctx& c, Amg& H, AmgLevel& V, const LvView<T>& W, const T* b, T* x, bool forward) {
+  const int nf = H.nf;
+  const int nfp = nf == 1 ? 1 : nf == 2 ? 2 : 4;
+  const int s = std::max(1, V.reach) + 1;
+  int nc[3];
+  for (int d = 0; d < 3; ++d) nc[d] = V.N[d] > 1 ? s : 1;
+  const int ncolour = nc[0] * nc[1] * nc[2];
+  const int lanes = std::min(V.lanes, 64 / nfp);
+  for (int k = 0; k < ncolour; ++k) {
+    const int col = forward ? k : ncolour - 1 - k;
+    const int cx = col % nc[0], cy = (col / nc[0]) % nc[1], cz = col / (nc[0] * nc[1]);
+    const int co[3] = {cx, cy, cz};
+    long cnt[3];
+    for (int d = 0; d < 3; ++d) cnt[d] = V.N[d] > co[d] ? (V.N[d] - co[d] + s - 1) / s : 0;
+    const long ncol = cnt[0] * cnt[1] * cnt[2];
+    if (!ncol) continue;
+    by_lanes(lanes, [&](auto LN) {
+      constexpr int K = decltype(LN)::value;
+      auto go = [&](auto NP) {
+        constexpr int F = decltype(NP)::value;
+        if constexpr (K * F <= 64)
+          hipLaunchKernelGGL((k_amg_gs<K, F, T>), g1(ncol * K * F), dim3(256), 0, c.stream, ncol, nf, s, cx, cy, cz,
+                             cnt[0], cnt[1], (long)V.N[0], (long)V.N[1], V.rp, V.ci, W.A, W.binv, b, x);
+      };
+      if (nfp == 1) go(std::integral_constant<int, 1>{});
+      else if (nfp == 2) go(std::integral_constant<int, 2>{});
+      else go(std::integral_constant<int, 4>{});
+    });
+  }
+}
+
 template <class T>
 static void vcycle(Ctx& c, Amg& H, const T* A0, const double* cinv, const T* v, T* out) {
   const int nl = (int)H.lv.size();
@@ -915,9 +1116,16 @@ static void vcycle(Ctx& c, Amg& H, const T* A0, const double* cinv, const T* v, 
     const LvView<T> W = view<T>(H, l, A0);
     const T* b = l ? W.b : v;
     T* x = l ? W.x : out;
-    const Cheb k = cheb_coef(V.lambda);
-    cheb_step<T>(c, V, W, b, x, W.t, 0.0, k.inv_theta, 1);
-    cheb_step<T>(c, V, W, b, W.t, x, k.a2, k.b2, 0);
+    if (H.smoother == kSmootherBlockJacobi) {
+      bj_smooth<T>(c, H, V, W, b, x, 1);
+    } else if (H.smoother == kSmootherBlockGaussSeidel) {
+      hipLaunchKernelGGL(k_amg_fill0<T>, g1(V.n), dim3(256), 0, c.stream, V.n, x);
+      gs_sweep<T>(c, H, V, W, b, x, true);
+    } else {
+      const Cheb k = cheb_coef(V.lambda);
+      cheb_step<T>(c, V, W, b, x, W.t, 0.0, k.inv_theta, 1);
+      cheb_step<T>(c, V, W, b, W.t, x, k.a2, k.b2, 0);
+    }
     by_lanes(V.lanes, [&](auto LN) {
       constexpr int K = decltype(LN)::value;
       hipLaunchKernelGGL((k_amg_resid<K, T>), g1(V.n * K), dim3(256), 0, c.stream, V.n, V.rp, V.ci, W.A, b, x, W.r);
@@ -950,9 +1158,15 @@ static void vcycle(Ctx& c, Amg& H, const T* A0, const double* cinv, const T* v, 
       hipLaunchKernelGGL((k_amg_spmv<K, T>), g1(V.n * K), dim3(256), 0, c.stream, V.n, V.prp.p, V.pci.p, W.P, Wc.x,
                          x, 1);
     });
-    const Cheb k = cheb_coef(V.lambda);
-    cheb_step<T>(c, V, W, b, x, W.t, 0.0, k.inv_theta, 0);
-    cheb_step<T>(c, V, W, b, W.t, x, k.a2, k.b2, 0);
+    if (H.smoother == kSmootherBlockJacobi) {
+      bj_smooth<T>(c, H, V, W, b, x, 0);
+    } else if (H.smoother == kSmootherBlockGaussSeidel) {
+      gs_sweep<T>(c, H, V, W, b, x, false);
+    } else {
+      const Cheb k = cheb_coef(V.lambda);
+      cheb_step<T>(c, V, W, b, x, W.t, 0.0, k.inv_theta, 0);
+      cheb_step<T>(c, V, W, b, W.t, x, k.a2, k.b2, 0);
+    }
   }
   MHX_HIP(hipGetLastError());
 }

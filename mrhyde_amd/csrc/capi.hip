@@ -849,6 +849,14 @@ int mhx_set_option(mhx_ctx* ctx, const char* name, int64_t value) {
     ctx->spmv_lanes = (int)value;
     return MHX_OK;
   }
+  if (k == "amg.smoother") {
+    if (value < 0 || value > 2) {
+      ctx->err = "amg.smoother must be 0 (Chebyshev), 1 (nodal block Jacobi) or 2 (nodal block Gauss-Seidel)";
+      return MHX_ERR_ARG;
+    }
+    ctx->amg_smoother = (int)value;
+    return MHX_OK;
+  }
   ctx->err = "unknown option " + k;
   return MHX_ERR_ARG;
 }
@@ -862,6 +870,10 @@ int mhx_get_option(const mhx_ctx* ctx, const char* name, int64_t* value) {
   }
   if (k == "krylov.spmv_lanes") {
     *value = ctx->spmv_lanes;
+    return MHX_OK;
+  }
+  if (k == "amg.smoother") {
+    *value = ctx->amg_smoother;
     return MHX_OK;
   }
   return MHX_ERR_ARG;

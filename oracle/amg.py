@@ -313,22 +313,93 @@ def cheb_coef(lam):
     return 1.0 / theta, rho1 * rho0, 2.0 * rho1 / delta
 
 
-def vcycle(levels, b):
-    """x = M^-1 b (Chebyshev(2) pre and post, exact coarsest solve)."""
+BJ_OMEGA = 0.6
+BJ_SWEEPS = 2
+
+
+def block_inverses(A: sp.csr_matrix, nf: int):
+    """Inverse of every node's nf x nf diagonal block (k_amg_block_inv); a
+    singular block gives a zero block.  Returns an array [n_nodes, nf, nf]."""
+    n = A.shape[0] // nf
+    out = np.zeros((n, nf, nf))
+    A = A.tocsr()
+    for k in range(n):
+        blk = A[k * nf:(k + 1) * nf, k * nf:(k + 1) * nf].toarray()
+        try:
+            out[k] = np.linalg.inv(blk)
+        except np.linalg.LinAlgError:
+            pass
+    return out
+
+
+def node_colours(N, reach):
+    """Colour of every lattice node for the multicolour sweep: the node's
+    coordinates modulo s = reach + 1, numbered x fastest over the axes that
+    have more than one node (gs_sweep)."""
+    s = max(1, reach) + 1
+    nc = [s if N[d] > 1 else 1 for d in range(3)]
+    idx = np.arange(N[0] * N[1] * N[2])
+    x, y, z = idx % N[0], (idx // N[0]) % N[1], idx // (N[0] * N[1])
+    return (x % s) + nc[0] * ((y % s) + nc[1] * (z % s)), nc[0] * nc[1] * nc[2]
+
+
+def block_apply(binv, r, nf):
+    return np.einsum("kfg,kg->kf", binv, r.reshape(-1, nf)).reshape(-1)
+
+
+def gs_sweep(A, binv, nf, N, reach, b, x, forward):
+    """One multicolour nodal block Gauss-Seidel sweep, in place on x: colour
+    by colour, x_S += Binv_S (b - A x)_S for the rows S of the colour's nodes."""
+    col, ncol = node_colours(N, reach)
+    rows_of = np.repeat(col, nf)
+    order = range(ncol) if forward else range(ncol - 1, -1, -1)
+    for c in order:
+        S = np.nonzero(rows_of == c)[0]
+        if S.size == 0:
+            continue
+        r = b[S] - A[S] @ x
+        nodes = S[::nf] // nf
+        x[S] += np.einsum("kfg,kg->kf", binv[nodes], r.reshape(-1, nf)).reshape(-1)
+    return x
+
+
+def vcycle(levels, b, smoother="chebyshev", nf=1):
+    """x = M^-1 b, exact coarsest solve; pre/post smoother "chebyshev"
+    (degree 2 on D^-1 A), "block_jacobi" (two damped sweeps each side) or
+    "block_gauss_seidel" (one multicolour sweep, forward before the coarse
+    correction and backward after it, so the cycle is symmetric for
+    symmetric A)."""
     def rec(l, b):
         L = levels[l]
         if l == len(levels) - 1:
             return L["Cinv"] @ b
         A, dinv = L["A"], L["dinv"]
-        it, a2, b2 = cheb_coef(L["lam"])
-        d = it * dinv * b
-        x = d.copy()
-        d = a2 * d + b2 * dinv * (b - A @ x)
-        x = x + d
+        b = np.asarray(b, dtype=np.float64)
+        if smoother == "chebyshev":
+            it, a2, b2 = cheb_coef(L["lam"])
+            d = it * dinv * b
+            x = d.copy()
+            d = a2 * d + b2 * dinv * (b - A @ x)
+            x = x + d
+        else:
+            if "binv" not in L:
+                L["binv"] = block_inverses(A, nf)
+            if smoother == "block_jacobi":
+                x = BJ_OMEGA * block_apply(L["binv"], b, nf)
+                for _ in range(BJ_SWEEPS - 1):
+                    x = x + BJ_OMEGA * block_apply(L["binv"], b - A @ x, nf)
+            else:
+                x = gs_sweep(A, L["binv"], nf, L["N"], L["reach"], b, np.zeros_like(b), True)
         xc = rec(l + 1, L["R"] @ (b - A @ x))
         x = x + L["P"] @ xc
-        d = it * dinv * (b - A @ x)
-        x = x + d
-        d = a2 * d + b2 * dinv * (b - A @ x)
-        return x + d
+        if smoother == "chebyshev":
+            d = it * dinv * (b - A @ x)
+            x = x + d
+            d = a2 * d + b2 * dinv * (b - A @ x)
+            return x + d
+        if smoother == "block_jacobi":
+            for _ in range(BJ_SWEEPS):
+                x = x + BJ_OMEGA * block_apply(L["binv"], b - A @ x, nf)
+            return x
+        return gs_sweep(A, L["binv"], nf, L["N"], L["reach"], b, x, False)
     return rec(0, b)
