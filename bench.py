@@ -162,6 +162,21 @@ def ns_bench(n: int, steps: int, newton: bool = True):
                         "mean_launch_ms": round(per_launch, 5),
                         "achieved_GBps": round(B / lpl / (per_launch * 1e-3) / 1e9, 1),
                         "frac": round(B / lpl / (per_launch * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}}
+    # the first Newton system (J, res at u0, resident) solved to 1e-10 from du = 0
+    from mrhyde_amd.abi import MHX_BUF_DU
+
+    lin = {}
+    for name, sm, prec, m in (("block_jacobi", 0, "block jacobi", 150), ("amg_block_jacobi_smoother", 1, "amg", 150)):
+        g.set_option("amg.smoother", sm)
+        g.state_upload(MHX_BUF_DU, np.zeros(g.n_dof))
+        g.synchronize()
+        t0 = time.perf_counter()
+        st = g.solve_resident(solver="gmres", restart=m, tol=1e-10, max_iters=3000, precond=prec)
+        g.synchronize()
+        lin[name] = {"solver": f"GMRES({m})", "iterations": st["iters"], "rel_res": st["rel_res"],
+                     "wall_s": round(time.perf_counter() - t0, 3)}
+    g.set_option("amg.smoother", 0)
+    out["first_newton_system"] = lin
     if newton:
         kw = dict(max_iters=10, nl_tol=1e-8, solver="gmres", restart=50, tol=1e-10, linear_max_iters=5000)
         res = {}

@@ -16,6 +16,20 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "slow: full-size configuration")
 
 
+def pytest_collection_finish(session):
+    """PyTorch ships its own HIP runtime beside the system one that libmhx.so
+    links.  When libmhx.so brings the device up first, torch reports no device
+    for the rest of the process, so the GPU tests that hand torch tensors to
+    the engine need torch's runtime started before any other test runs."""
+    if not any(item.get_closest_marker("gpu") for item in session.items):
+        return
+    try:
+        import torch
+    except ImportError:
+        return
+    torch.cuda.is_available()
+
+
 def spmv_np(rowptr, colidx, vals, x):
     """Reference CSR product in numpy (checker only)."""
     y = np.zeros(len(rowptr) - 1)
