@@ -36,7 +36,7 @@ LDS_PEAK_TBS = 256 * 128 * 2.4e9 / 1e12  # 256 CUs x 128 B/clk x 2.4 GHz
 # 8 colour launches), scaled by the factors that make the same counters read
 # the known size of the plain copy kernel: FETCH x 2048 B, WRITE x 1024 B.
 # Source: profiles/k_thermal_q2hex_96cubed.md.  Only valid for that workload.
-PMC_TRAFFIC = {(96, 0): {"fetch_kib": 529029.4, "write_kib": 1037174.3, "fetch_scale": 2048, "write_scale": 1024}}
+PMC_TRAFFIC = {(96, 0): {"fetch_kib": 529028.5, "write_kib": 1032647.0, "fetch_scale": 2048, "write_scale": 1024}}
 
 
 def c2_problem(n: int):
