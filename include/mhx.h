@@ -309,12 +309,17 @@ int  mhx_l2_error(mhx_ctx* ctx, const double* u, int32_t n_modes, const double* 
  *                         chosen from the mean row length at context build).
  *   "amg.smoother"        smoother of the AMG V-cycle: 0 = degree-2
  *                         Chebyshev on the point diagonal (default), 1 =
- *                         two sweeps of damped (0.6) Jacobi on the nodal
- *                         n_field x n_field diagonal blocks, for coupled
- *                         multi-field systems (the reference's MueLu
+ *                         damped Jacobi sweeps on the nodal n_field x
+ *                         n_field diagonal blocks, 2 = one multicolour
+ *                         nodal block Gauss-Seidel sweep each side; 1 and
+ *                         2 are for coupled multi-field systems (the reference's MueLu
  *                         "smoother: type" key, CHEBYSHEV or RELAXATION,
  *                         src/interfaces/linear_algebra/
  *                         linearAlgebraInterface_solvers.hpp:225).
+ *   "amg.sweeps"          block-Jacobi smoother sweeps before and after
+ *                         the coarse correction, 1 .. 16 (default 3).
+ *   "amg.omega_pct"       block-Jacobi damping in percent, 1 .. 200
+ *                         (default 40).
  * Returns MHX_ERR_ARG for an unknown name or value. */
 int  mhx_set_option(mhx_ctx* ctx, const char* name, int64_t value);
 int  mhx_get_option(const mhx_ctx* ctx, const char* name, int64_t* value);

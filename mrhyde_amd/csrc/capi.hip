@@ -849,6 +849,22 @@ int mhx_set_option(mhx_ctx* ctx, const char* name, int64_t value) {
     ctx->spmv_lanes = (int)value;
     return MHX_OK;
   }
+  if (k == "amg.sweeps") {
+    if (value < 1 || value > 16) {
+      ctx->err = "amg.sweeps must be 1 .. 16";
+      return MHX_ERR_ARG;
+    }
+    ctx->amg_sweeps = (int)value;
+    return MHX_OK;
+  }
+  if (k == "amg.omega_pct") {
+    if (value < 1 || value > 200) {
+      ctx->err = "amg.omega_pct must be 1 .. 200";
+      return MHX_ERR_ARG;
+    }
+    ctx->amg_omega_pct = (int)value;
+    return MHX_OK;
+  }
   if (k == "amg.smoother") {
     if (value < 0 || value > 2) {
       ctx->err = "amg.smoother must be 0 (Chebyshev), 1 (nodal block Jacobi) or 2 (nodal block Gauss-Seidel)";
@@ -870,6 +886,14 @@ int mhx_get_option(const mhx_ctx* ctx, const char* name, int64_t* value) {
   }
   if (k == "krylov.spmv_lanes") {
     *value = ctx->spmv_lanes;
+    return MHX_OK;
+  }
+  if (k == "amg.sweeps") {
+    *value = ctx->amg_sweeps;
+    return MHX_OK;
+  }
+  if (k == "amg.omega_pct") {
+    *value = ctx->amg_omega_pct;
     return MHX_OK;
   }
   if (k == "amg.smoother") {

@@ -313,8 +313,8 @@ def cheb_coef(lam):
     return 1.0 / theta, rho1 * rho0, 2.0 * rho1 / delta
 
 
-BJ_OMEGA = 0.6
-BJ_SWEEPS = 2
+BJ_OMEGA = 0.4
+BJ_SWEEPS = 3
 
 
 def block_inverses(A: sp.csr_matrix, nf: int):
@@ -365,7 +365,8 @@ def gs_sweep(A, binv, nf, N, reach, b, x, forward):
 
 def vcycle(levels, b, smoother="chebyshev", nf=1):
     """x = M^-1 b, exact coarsest solve; pre/post smoother "chebyshev"
-    (degree 2 on D^-1 A), "block_jacobi" (two damped sweeps each side) or
+    (degree 2 on D^-1 A), "block_jacobi" (BJ_SWEEPS sweeps damped by
+    BJ_OMEGA on each side) or
     "block_gauss_seidel" (one multicolour sweep, forward before the coarse
     correction and backward after it, so the cycle is symmetric for
     symmetric A)."""

@@ -302,3 +302,31 @@ def test_block_smoother_amg_solve(name, solver, sm):
         x, st = g.linearSolver(j, r, opts=krylov_opts(solver=solver, tol=1e-10, max_iters=500, precond=prec))
         rel = np.linalg.norm(r - A @ x) / np.linalg.norm(r)
         assert st["rel_res"] <= 1e-10 and rel <= 1.5e-10, (prec, st, rel)
+
+
+@pytest.mark.gpu
+def test_ns_newton_with_block_jacobi_amg():
+    """Newton on a 3-D lid-driven cavity (Re 100) with GMRES(50) and the AMG
+    cycle smoothed by nodal block Jacobi: every linear solve reaches 1e-10 and
+    the solution equals a Newton iteration on the oracle's residual and
+    Jacobian with sparse LU solves."""
+    from mrhyde_amd import Mhx
+    from oracle import Oracle
+
+    p = Problem(dim=3, order=1, quadrature=2, n=(10, 10, 10), physics=MHX_PHYS_NAVIER_STOKES, mu=0.01, supg=True,
+                pspg=True, dbc_sides=1 | 2 | 4 | 16 | 32, lid_sides=8, lid_u=(1.0, 0.0, 0.0), pin_pressure=True)
+    o = Oracle(p)
+    u0 = o.setDirichlet(1e-3 * np.random.default_rng(3).standard_normal(o.n_dof))
+    rp, ci = o.crs_pattern()
+    u_ref = u0.copy()
+    for _ in range(6):
+        r, j = o.assembleJacRes(u_ref)
+        u_ref = u_ref + spla.spsolve(sp.csr_matrix((j, ci, rp), shape=(o.n_dof, o.n_dof)).tocsc(), r)
+    g = Mhx(p)
+    g.set_option("amg.smoother", 1)
+    print("[amg-newton] 10^3 cavity", flush=True)
+    u, st = g.nonlinearSolver(u0.copy(), max_iters=10, nl_tol=1e-12, solver="gmres", restart=50, tol=1e-10,
+                              linear_max_iters=500, precond="amg")
+    assert st["rel_res"] <= 1e-10 and st["res_norm"] <= 1e-12 * st["res_norm0"], st
+    assert st["iters"] <= 50 * st["nl_iters"], st
+    np.testing.assert_allclose(u, u_ref, rtol=0, atol=1e-9 * np.abs(u_ref).max())
