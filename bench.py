@@ -120,6 +120,19 @@ def krylov_bench(g, iters: int):
                                 "setup_ms": round(setup_ms, 1), "vcycle_ms": round(vc_ms / max(vc_n, 1), 3),
                                 "levels": [g.amg_level(lv, "A")[1]["rows"]
                                            for lv in range(g.amg_level(0, "A")[1]["levels"])]}
+    # the same with the FP32 hierarchy and inner GMRES in FP64 refinement
+    x64 = g.state_download(MHX_BUF_DU)
+    g.state_upload(MHX_BUF_DU, np.zeros(N))
+    g.synchronize()
+    t0 = time.perf_counter()
+    st = g.solve_resident(solver="gmres", restart=m, tol=1e-10, max_iters=1000, precond="amg-fp32")
+    g.synchronize()
+    dt = time.perf_counter() - t0
+    x32 = g.state_download(MHX_BUF_DU)
+    out["amg_fp32_refined_to_1e-10"] = {"iterations": st["iters"], "refinements": st["refinements"],
+                                        "rel_res": st["rel_res"], "solve_latency_ms": round(dt * 1e3, 1),
+                                        "max_abs_diff_vs_fp64": float(np.abs(x32 - x64).max()),
+                                        "max_abs_fp64": float(np.abs(x64).max())}
     return out
 
 
@@ -299,6 +312,10 @@ def serving_bench(n_req: int = 240, rate: float = 200.0, seed: int = 5, per_call
     srv = Server(max_active=8, max_cached=48, solver="cg", precond="amg", tol=1e-10)
     out["saturated_amg"] = run(srv, np.zeros(n_req))
     srv.close()
+    # mixed precision: FP32 hierarchy and inner CG, FP64 residual and refinement
+    srv = Server(max_active=8, max_cached=48, solver="cg", precond="amg-fp32", tol=1e-10)
+    out["saturated_amg_fp32_refined"] = run(srv, np.zeros(n_req))
+    srv.close()
     idx = np.linspace(0, n_req - 1, per_call_sample).astype(int)
     t_all = 0.0
     for i in idx:
@@ -318,6 +335,10 @@ def serving_bench(n_req: int = 240, rate: float = 200.0, seed: int = 5, per_call
     out["speedup_served_vs_per_call"] = round(best / out["per_call"]["solves_per_s"], 2)
     out["speedup_amg_vs_jacobi_served"] = round(out["saturated_amg"]["solves_per_s"] /
                                                 out["saturated_fp64"]["solves_per_s"], 2)
+    # the served mixed-precision AMG form against the plain per-call path
+    # (Jacobi-CG, fresh context per request), both to 1e-10
+    out["speedup_served_amg_fp32r_vs_per_call"] = round(out["saturated_amg_fp32_refined"]["solves_per_s"] /
+                                                        out["per_call"]["solves_per_s"], 2)
     return out
 
 

@@ -708,7 +708,10 @@ static int mixed_solve(Ctx& c, const double* A, const double* b, double* x, cons
     *relres = 0.0;
     return 0;
   }
-  const double inner_tol = o.inner_tol > 0 ? o.inner_tol : 1e-4;
+  // automatic inner target: 1e-4, or 1e-3 with the AMG cycle, whose FP32
+  // rounding stops the inner residual near 1e-4 on the 96^3 thermal and 64^3
+  // cavity systems (a 1e-4 target there runs the inner solve to its limit)
+  const double inner_tol = o.inner_tol > 0 ? o.inner_tol : ((o.precond & 0xff) == MHX_PREC_AMG ? 1e-3 : 1e-4);
   const int inner_max = o.inner_iters > 0 ? o.inner_iters : maxit;
   const int m = o.restart > 0 ? o.restart : 50;
   int total = 0, refs = 0, stalls = 0;
