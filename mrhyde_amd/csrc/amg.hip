@@ -340,46 +340,60 @@ __global__ __launch_bounds__(kAmgBlk) void k_amg_R(Geo G, const int32_t* __restr
   }
 }
 
-// AP(i, (J, g)) in a dense 3x3x3 aggregate box per (fine row i, coarse field g)
+// AP(i, (J, g)) in a dense 3x3x3 aggregate box per (fine row i, coarse field
+// g).  A group of kApLanes lanes owns one (i, g): lane l takes the row's
+// entries l, l + kApLanes, ... (coalesced reads of the CRS row), accumulates
+// its own 27-slot box in LDS, and the group then sums the boxes slot by slot
+// in lane order, so every set-up gives the same bits.
+constexpr int kApLanes = 8;
 __global__ __launch_bounds__(kAmgBlk) void k_amg_AP(Geo G, long n, const int32_t* __restrict__ rp,
                                                     const int32_t* __restrict__ ci, const double* __restrict__ A,
                                                     const int32_t* __restrict__ prp, const double* __restrict__ P,
                                                     double* __restrict__ AP, int* __restrict__ err) {
   __shared__ double acc_s[kAmgBlk][28];
   const int nf = G.nf;
-  const long t = (long)blockIdx.x * kAmgBlk + threadIdx.x;
-  if (t >= n * nf) return;
+  const int l = threadIdx.x % kApLanes;
+  const long t = ((long)blockIdx.x * kAmgBlk + threadIdx.x) / kApLanes;
+  const bool active = t < n * nf;
   double* acc = acc_s[threadIdx.x];
-  const long i = t / nf;
-  const int g = (int)(t - i * nf);
-  int px, py, pz;
-  node_xyz(i / nf, G.ax[0].n, G.ax[1].n, px, py, pz);
-  const int alo[3] = {G.ax[0].t[kAplo][px], G.ax[1].t[kAplo][py], G.ax[2].t[kAplo][pz]};
-  const int aex[3] = {G.ax[0].t[kApext][px], G.ax[1].t[kApext][py], G.ax[2].t[kApext][pz]};
   for (int s = 0; s < 27; ++s) acc[s] = 0.0;
-  for (int k = rp[i]; k < rp[i + 1]; ++k) {
-    const long j = ci[k];
-    const double a = A[k];
-    int qx, qy, qz;
-    node_xyz(j / nf, G.ax[0].n, G.ax[1].n, qx, qy, qz);
-    const int plx = G.ax[0].t[kPlo][qx], ply = G.ax[1].t[kPlo][qy], plz = G.ax[2].t[kPlo][qz];
-    const int pex = G.ax[0].t[kPext][qx], pey = G.ax[1].t[kPext][qy], pez = G.ax[2].t[kPext][qz];
-    const long pb = prp[j];
-    for (int dz = 0; dz < pez; ++dz)
-      for (int dy = 0; dy < pey; ++dy)
-        for (int dx = 0; dx < pex; ++dx) {
-          const int sx = plx + dx - alo[0], sy = ply + dy - alo[1], sz = plz + dz - alo[2];
-          if (sx < 0 || sx >= aex[0] || sy < 0 || sy >= aex[1] || sz < 0 || sz >= aex[2]) {
-            atomicOr(err, 2);
-            continue;
+  const long i = active ? t / nf : 0;
+  const int g = active ? (int)(t - i * nf) : 0;
+  if (active) {
+    int px, py, pz;
+    node_xyz(i / nf, G.ax[0].n, G.ax[1].n, px, py, pz);
+    const int alo[3] = {G.ax[0].t[kAplo][px], G.ax[1].t[kAplo][py], G.ax[2].t[kAplo][pz]};
+    const int aex[3] = {G.ax[0].t[kApext][px], G.ax[1].t[kApext][py], G.ax[2].t[kApext][pz]};
+    for (int k = rp[i] + l; k < rp[i + 1]; k += kApLanes) {
+      const long j = ci[k];
+      const double a = A[k];
+      int qx, qy, qz;
+      node_xyz(j / nf, G.ax[0].n, G.ax[1].n, qx, qy, qz);
+      const int plx = G.ax[0].t[kPlo][qx], ply = G.ax[1].t[kPlo][qy], plz = G.ax[2].t[kPlo][qz];
+      const int pex = G.ax[0].t[kPext][qx], pey = G.ax[1].t[kPext][qy], pez = G.ax[2].t[kPext][qz];
+      const long pb = prp[j];
+      for (int dz = 0; dz < pez; ++dz)
+        for (int dy = 0; dy < pey; ++dy)
+          for (int dx = 0; dx < pex; ++dx) {
+            const int sx = plx + dx - alo[0], sy = ply + dy - alo[1], sz = plz + dz - alo[2];
+            if (sx < 0 || sx >= aex[0] || sy < 0 || sy >= aex[1] || sz < 0 || sz >= aex[2]) {
+              atomicOr(err, 2);
+              continue;
+            }
+            const double pv = P[pb + ((dz * pey + dy) * pex + dx) * nf + g];
+            double& r = acc[(sz * 3 + sy) * 3 + sx];
+            r = fma(a, pv, r);
           }
-          const double pv = P[pb + ((dz * pey + dy) * pex + dx) * nf + g];
-          double& r = acc[(sz * 3 + sy) * 3 + sx];
-          r = fma(a, pv, r);
-        }
+    }
   }
-  double* out = AP + t * 27;
-  for (int s = 0; s < 27; ++s) out[s] = acc[s];
+  __syncthreads();
+  if (!active) return;
+  const int g0 = threadIdx.x - l;  // first lane of the group
+  for (int s = l; s < 27; s += kApLanes) {
+    double v = 0.0;
+    for (int q = 0; q < kApLanes; ++q) v += acc_s[g0 + q][s];
+    AP[t * 27 + s] = v;
+  }
 }
 
 // A_c = R (A P): one workgroup per coarse row (I, f), each thread owning up to
@@ -941,7 +955,7 @@ void amg_setup(Ctx& c, const double* A, bool fp32) {
                        L->rci.p, L->R.p);
     // A P (dense 27-slot boxes), then A_c = R (A P)
     H.scratch.alloc(std::max<size_t>(H.scratch.n, (size_t)n * nf * 27));
-    hipLaunchKernelGGL(k_amg_AP, g1(n * nf, kAmgBlk), dim3(kAmgBlk), 0, c.stream, G, n, L->rp, L->ci, Acur,
+    hipLaunchKernelGGL(k_amg_AP, g1(n * nf * kApLanes, kAmgBlk), dim3(kAmgBlk), 0, c.stream, G, n, L->rp, L->ci, Acur,
                        L->prp.p, L->P.p, H.scratch.p, H.err.p);
     C.rp_own.alloc(nc + 1);
     C.ci_own.alloc(a_nnz);
