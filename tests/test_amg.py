@@ -330,3 +330,23 @@ def test_ns_newton_with_block_jacobi_amg():
     assert st["rel_res"] <= 1e-10 and st["res_norm"] <= 1e-12 * st["res_norm0"], st
     assert st["iters"] <= 50 * st["nl_iters"], st
     np.testing.assert_allclose(u, u_ref, rtol=0, atol=1e-9 * np.abs(u_ref).max())
+
+
+@pytest.mark.gpu
+def test_amg_options_are_validated():
+    """The AMG options keep their defaults until set, accept their documented
+    ranges and refuse anything else with MHX_ERR_ARG and a message."""
+    from mrhyde_amd import Mhx
+    from mrhyde_amd.engine import MhxError
+
+    g = Mhx(CASES["2d_q1_linear"][0])
+    assert (g.get_option("amg.smoother"), g.get_option("amg.sweeps"), g.get_option("amg.omega_pct")) == (0, 3, 40)
+    for name, good, bad in (("amg.smoother", (0, 1, 2), (-1, 3)), ("amg.sweeps", (1, 16), (0, 17)),
+                            ("amg.omega_pct", (1, 200), (0, 201))):
+        for v in good:
+            g.set_option(name, v)
+            assert g.get_option(name) == v
+        for v in bad:
+            with pytest.raises(MhxError, match=name.split(".")[1]):
+                g.set_option(name, v)
+            assert g.get_option(name) == good[-1]
