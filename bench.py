@@ -190,7 +190,7 @@ def ns_bench(n: int, steps: int, newton: bool = True):
         g.synchronize()
         lin[name] = {"solver": f"GMRES({m})", "iterations": st["iters"], "rel_res": st["rel_res"],
                      "wall_s": round(time.perf_counter() - t0, 3)}
-    g.set_option("amg.smoother", 0)
+    g.set_option("amg.smoother", -1)
     out["first_newton_system"] = lin
     if newton:
         kw = dict(max_iters=10, nl_tol=1e-8, solver="gmres", restart=50, tol=1e-10, linear_max_iters=2000)
@@ -200,7 +200,7 @@ def ns_bench(n: int, steps: int, newton: bool = True):
             t0 = time.perf_counter()
             u, st = g.nonlinearSolver(u0.copy(), precond=prec, **kw)
             res[name] = (u, st, time.perf_counter() - t0)
-        g.set_option("amg.smoother", 0)
+        g.set_option("amg.smoother", -1)
         u32, s32, t32 = res["fp32-refined"]
         u64, s64, t64 = res["fp64"]
         out["newton"] = {

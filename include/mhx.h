@@ -307,8 +307,10 @@ int  mhx_l2_error(mhx_ctx* ctx, const double* u, int32_t n_modes, const double* 
  *   "krylov.spmv_lanes"   lanes that share one CRS row in the sparse
  *                         matrix-vector product: 4, 8, 16, 32 or 64 (default
  *                         chosen from the mean row length at context build).
- *   "amg.smoother"        smoother of the AMG V-cycle: 0 = degree-2
- *                         Chebyshev on the point diagonal (default), 1 =
+ *   "amg.smoother"        smoother of the AMG V-cycle: -1 = automatic
+ *                         (default: 0 for one unknown per node, 1 for
+ *                         coupled fields), 0 = degree-2
+ *                         Chebyshev on the point diagonal, 1 =
  *                         damped Jacobi sweeps on the nodal n_field x
  *                         n_field diagonal blocks, 2 = one multicolour
  *                         nodal block Gauss-Seidel sweep each side; 1 and

@@ -885,7 +885,9 @@ void amg_setup(Ctx& c, const double* A, bool fp32) {
   MHX_HIP(hipMemsetAsync(H.err.p, 0, sizeof(int), c.stream));
   H.A0 = A;
   H.fp32 = fp32;
-  H.smoother = c.amg_smoother;
+  // automatic choice: Chebyshev on the point diagonal for one unknown per
+  // node, nodal block Jacobi for coupled fields
+  H.smoother = c.amg_smoother >= 0 ? c.amg_smoother : (nf > 1 ? kSmootherBlockJacobi : kSmootherChebyshev);
   H.sweeps = c.amg_sweeps;
   H.omega = c.amg_omega_pct * 0.01;
   H.nf = nf;

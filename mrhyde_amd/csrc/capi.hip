@@ -866,8 +866,8 @@ int mhx_set_option(mhx_ctx* ctx, const char* name, int64_t value) {
     return MHX_OK;
   }
   if (k == "amg.smoother") {
-    if (value < 0 || value > 2) {
-      ctx->err = "amg.smoother must be 0 (Chebyshev), 1 (nodal block Jacobi) or 2 (nodal block Gauss-Seidel)";
+    if (value < -1 || value > 2) {
+      ctx->err = "amg.smoother must be -1 (automatic), 0 (Chebyshev), 1 (nodal block Jacobi) or 2 (nodal block Gauss-Seidel)";
       return MHX_ERR_ARG;
     }
     ctx->amg_smoother = (int)value;

@@ -340,8 +340,8 @@ def test_amg_options_are_validated():
     from mrhyde_amd.engine import MhxError
 
     g = Mhx(CASES["2d_q1_linear"][0])
-    assert (g.get_option("amg.smoother"), g.get_option("amg.sweeps"), g.get_option("amg.omega_pct")) == (0, 3, 40)
-    for name, good, bad in (("amg.smoother", (0, 1, 2), (-1, 3)), ("amg.sweeps", (1, 16), (0, 17)),
+    assert (g.get_option("amg.smoother"), g.get_option("amg.sweeps"), g.get_option("amg.omega_pct")) == (-1, 3, 40)
+    for name, good, bad in (("amg.smoother", (-1, 0, 1, 2), (-2, 3)), ("amg.sweeps", (1, 16), (0, 17)),
                             ("amg.omega_pct", (1, 200), (0, 201))):
         for v in good:
             g.set_option(name, v)
