@@ -1121,6 +1121,8 @@ static void gs_sweep(Ctx& c, Amg& H, AmgLevel& V, const LvView<T>& W, const T* b
         if constexpr (K * F <= 64)
           hipLaunchKernelGGL((k_amg_gs<K, F, T>), g1(ncol * K * F), dim3(256), 0, c.stream, ncol, nf, s, cx, cy, cz,
                              cnt[0], cnt[1], (long)V.N[0], (long)V.N[1], V.rp, V.ci, W.A, W.binv, b, x);
+        else
+          throw Error{MHX_ERR_STATE, "AMG Gauss-Seidel: node group wider than a wavefront"};
       };
       if (nfp == 1) go(std::integral_constant<int, 1>{});
       else if (nfp == 2) go(std::integral_constant<int, 2>{});
