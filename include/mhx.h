@@ -161,7 +161,7 @@ typedef struct mhx_krylov_opts {
   int32_t restart;       /* "Num Blocks" (GMRES) */
   double  tol;           /* "linear TOL": ||b - A x||_2 / ||b||_2 <= tol */
   int32_t flags;         /* MHX_DEVICE_PTRS */
-  int32_t inner_iters;   /* FP32 inner iterations per refinement sweep (0 = auto) */
+  int32_t inner_iters;   /* FP32 inner iterations per refinement sweep (0 = auto: 10 restart cycles, >= 500) */
   double  inner_tol;     /* FP32 inner relative tolerance (0 = auto: 1e-4, 1e-3 with AMG) */
 } mhx_krylov_opts;
 
