@@ -232,6 +232,12 @@ def c2_newton_bench(n: int):
     wall = time.perf_counter() - t0
     prof = g.profile_read()
     g.profile(False)
+    # the same with the FP32 hierarchy and inner GMRES in FP64 refinement
+    u32 = g.setDirichlet(np.zeros(g.n_dof))
+    t0 = time.perf_counter()
+    u32, st32 = g.nonlinearSolver(u32, max_iters=5, nl_tol=1e-12, backtracking=False, solver="gmres", restart=50,
+                                  tol=1e-10, linear_max_iters=1000, precond="amg-fp32")
+    wall32 = time.perf_counter() - t0
     g.close()
     ph = {k: round(v[0], 1) for k, v in prof.items()
           if k in ("AssemblyManager::assembleJacRes", "LinearAlgebraInterface::linearSolver",
@@ -240,7 +246,12 @@ def c2_newton_bench(n: int):
                         "GMRES(50) + AMG to 1e-10 per step",
             "wall_s": round(wall, 3), "nl_iters": st["nl_iters"], "krylov_iters": st["iters"],
             "res_norm0": st["res_norm0"], "res_norm": st["res_norm"], "last_linear_rel_res": st["rel_res"],
-            "phases_ms": ph}
+            "phases_ms": ph,
+            "fp32_refined": {"wall_s": round(wall32, 3), "nl_iters": st32["nl_iters"], "krylov_iters": st32["iters"],
+                             "refinements": st32["refinements"], "res_norm": st32["res_norm"],
+                             "last_linear_rel_res": st32["rel_res"],
+                             "max_abs_diff_vs_fp64": float(np.abs(u32 - u).max()),
+                             "max_abs_u": float(np.abs(u).max())}}
 
 
 def c5_requests(count: int, seed: int):
