@@ -35,6 +35,7 @@
 #include <memory>
 #include <type_traits>
 
+#include "lattice.hpp"
 #include "mhx_internal.hpp"
 
 namespace mhx {
@@ -637,9 +638,9 @@ __global__ void k_amg_bj(long n, int nf, const T* __restrict__ binv, const T* __
 // ---------------------------------------------------------------------------
 // V-cycle kernels (T = double or float; sums in FP64)
 // ---------------------------------------------------------------------------
-template <int LANES, class T>
-__device__ __forceinline__ double row_dot(const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
-                                          const T* __restrict__ A, const T* __restrict__ x, long row, int l) {
+template <int LANES, class T, class CI>
+__device__ __forceinline__ double row_dot(const int32_t* __restrict__ rp, CI ci, const T* __restrict__ A,
+                                          const T* __restrict__ x, long row, int l) {
   const int e = rp[row + 1];
   double acc = 0.0;
   int k = rp[row] + l;
@@ -649,12 +650,12 @@ __device__ __forceinline__ double row_dot(const int32_t* __restrict__ rp, const 
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       av[u] = A[k + u * LANES];
-      cv[u] = ci[k + u * LANES];
+      cv[u] = ci(k + u * LANES);
     }
 #pragma unroll
     for (int u = 0; u < 4; ++u) acc = fma((double)av[u], (double)x[cv[u]], acc);
   }
-  for (; k < e; k += LANES) acc = fma((double)A[k], (double)x[ci[k]], acc);
+  for (; k < e; k += LANES) acc = fma((double)A[k], (double)x[ci(k)], acc);
 #pragma unroll
   for (int off = LANES / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, LANES);
   return acc;
@@ -662,15 +663,25 @@ __device__ __forceinline__ double row_dot(const int32_t* __restrict__ rp, const 
 
 // one Chebyshev step: r = b - A xin (A xin skipped when zero_x),
 // d = alpha d + beta D^-1 r, xout = xin + d (xin = 0 when zero_x)
-template <int LANES, class T>
-__global__ __launch_bounds__(256) void k_amg_cheb(long n, const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
-                                                  const T* __restrict__ A, const T* __restrict__ b,
-                                                  const T* __restrict__ xin, T* __restrict__ xout, T* __restrict__ d,
+// LAT: the level-0 operator in the context's lattice pattern, columns from its
+// closed form instead of ci (same entries, same order, same sums)
+template <bool LAT>
+__device__ __forceinline__ auto level_cols(const BoxMesh& m, const int32_t* __restrict__ rp,
+                                           const int32_t* __restrict__ ci, long row) {
+  if constexpr (LAT) return LatCols{lat_row(m, row), rp[row]};
+  else return CsrCols{ci};
+}
+
+template <int LANES, bool LAT, class T>
+__global__ __launch_bounds__(256) void k_amg_cheb(BoxMesh m, long n, const int32_t* __restrict__ rp,
+                                                  const int32_t* __restrict__ ci, const T* __restrict__ A,
+                                                  const T* __restrict__ b, const T* __restrict__ xin,
+                                                  T* __restrict__ xout, T* __restrict__ d,
                                                   const T* __restrict__ dinv, double alpha, double beta, int zero_x) {
   const long row = ((long)blockIdx.x * blockDim.x + threadIdx.x) / LANES;
   const int l = threadIdx.x % LANES;
   if (row >= n) return;
-  const double ax = zero_x ? 0.0 : row_dot<LANES, T>(rp, ci, A, xin, row, l);
+  const double ax = zero_x ? 0.0 : row_dot<LANES, T>(rp, level_cols<LAT>(m, rp, ci, row), A, xin, row, l);
   if (l == 0) {
     const double r = (double)b[row] - ax;
     const double dn = (alpha != 0.0 ? alpha * (double)d[row] : 0.0) + beta * (double)dinv[row] * r;
@@ -680,15 +691,15 @@ __global__ __launch_bounds__(256) void k_amg_cheb(long n, const int32_t* __restr
 }
 
 // r = b - A x
-template <int LANES, class T>
-__global__ __launch_bounds__(256) void k_amg_resid(long n, const int32_t* __restrict__ rp,
+template <int LANES, bool LAT, class T>
+__global__ __launch_bounds__(256) void k_amg_resid(BoxMesh m, long n, const int32_t* __restrict__ rp,
                                                    const int32_t* __restrict__ ci, const T* __restrict__ A,
                                                    const T* __restrict__ b, const T* __restrict__ x,
                                                    T* __restrict__ r) {
   const long row = ((long)blockIdx.x * blockDim.x + threadIdx.x) / LANES;
   const int l = threadIdx.x % LANES;
   if (row >= n) return;
-  const double ax = row_dot<LANES, T>(rp, ci, A, x, row, l);
+  const double ax = row_dot<LANES, T>(rp, level_cols<LAT>(m, rp, ci, row), A, x, row, l);
   if (l == 0) r[row] = (T)((double)b[row] - ax);
 }
 
@@ -717,7 +728,7 @@ __global__ __launch_bounds__(256) void k_amg_gs(long ncol, int nf, int s, int cx
     const long a = t % cntx, q = t / cntx;
     node = (cx + s * a) + Nx * ((cy + s * (q % cnty)) + Ny * (cz + s * (q / cnty)));
     row = node * nf + f;
-    res = (double)b[row] - row_dot<LANES, T>(rp, ci, A, x, row, l);
+    res = (double)b[row] - row_dot<LANES, T>(rp, CsrCols{ci}, A, x, row, l);
   }
   const int base = (int)(threadIdx.x & 63) - within;
   double upd = 0.0;
@@ -743,7 +754,7 @@ __global__ __launch_bounds__(256) void k_amg_spmv(long n, const int32_t* __restr
   const long row = ((long)blockIdx.x * blockDim.x + threadIdx.x) / LANES;
   const int l = threadIdx.x % LANES;
   if (row >= n) return;
-  const double v = row_dot<LANES, T>(rp, ci, M, x, row, l);
+  const double v = row_dot<LANES, T>(rp, CsrCols{ci}, M, x, row, l);
   if (l == 0) y[row] = (T)(add ? (double)y[row] + v : v);
 }
 
@@ -784,6 +795,7 @@ struct AmgLevel {
   DevBuf<int32_t> prp, pci, rrp, rci, tab;
   DevBuf<double> P, R;
   DevBuf<double> binv;  // nodal nf x nf diagonal-block inverses (block-Jacobi smoother)
+  bool lat = false;     // level 0 in the context's lattice pattern: closed-form columns
   double lambda = 1.0;
   AmgVals<float> f32;
   AmgVals<double> f64;  // FP64 work vectors (values live in A/P/R above)
@@ -906,6 +918,7 @@ void amg_setup(Ctx& c, const double* A, bool fp32) {
   L->rp = c.rowptr.p;
   L->ci = c.colidx.p;
   L->lanes = c.spmv_lanes;
+  L->lat = c.lattice_cols != 0;
   L->dinv.alloc(L->n);
   L->ns.alloc(L->n);
   hipLaunchKernelGGL(k_amg_diag, g1(L->n), dim3(256), 0, c.stream, L->n, L->rp, L->ci, A, L->dinv.p);
@@ -1067,13 +1080,35 @@ LvView<float> view<float>(Amg& H, int l, const float* A0) {
           V.f32.r.p, V.f32.d.p};
 }
 
+template <class F>
+static void by_lat(bool lat, F&& f) {
+  if (lat) f(std::true_type{});
+  else f(std::false_type{});
+}
+
 template <class T>
 static void cheb_step(Ctx& c, AmgLevel& V, const LvView<T>& W, const T* b, const T* xin, T* xout, double alpha,
                       double beta, int zero_x) {
   by_lanes(zero_x ? 4 : V.lanes, [&](auto LN) {
-    constexpr int K = decltype(LN)::value;
-    hipLaunchKernelGGL((k_amg_cheb<K, T>), g1(V.n * K), dim3(256), 0, c.stream, V.n, V.rp, V.ci, W.A, b, xin, xout,
-                       W.d, W.dinv, alpha, beta, zero_x);
+    by_lat(V.lat, [&](auto LT) {
+      constexpr int K = decltype(LN)::value;
+      constexpr bool LAT = decltype(LT)::value;
+      hipLaunchKernelGGL((k_amg_cheb<K, LAT, T>), g1(V.n * K), dim3(256), 0, c.stream, c.mesh, V.n, V.rp, V.ci, W.A,
+                         b, xin, xout, W.d, W.dinv, alpha, beta, zero_x);
+    });
+  });
+}
+
+// r = b - A x on one level
+template <class T>
+static void level_resid(Ctx& c, AmgLevel& V, const LvView<T>& W, const T* b, const T* x, T* r) {
+  by_lanes(V.lanes, [&](auto LN) {
+    by_lat(V.lat, [&](auto LT) {
+      constexpr int K = decltype(LN)::value;
+      constexpr bool LAT = decltype(LT)::value;
+      hipLaunchKernelGGL((k_amg_resid<K, LAT, T>), g1(V.n * K), dim3(256), 0, c.stream, c.mesh, V.n, V.rp, V.ci,
+                         W.A, b, x, r);
+    });
   });
 }
 
@@ -1084,10 +1119,7 @@ static void bj_smooth(Ctx& c, Amg& H, AmgLevel& V, const LvView<T>& W, const T* 
   for (int s = 0; s < H.sweeps; ++s) {
     const T* r = b;
     if (!(zero_x && s == 0)) {
-      by_lanes(V.lanes, [&](auto LN) {
-        constexpr int K = decltype(LN)::value;
-        hipLaunchKernelGGL((k_amg_resid<K, T>), g1(V.n * K), dim3(256), 0, c.stream, V.n, V.rp, V.ci, W.A, b, x, W.t);
-      });
+      level_resid<T>(c, V, W, b, x, W.t);
       r = W.t;
     }
     hipLaunchKernelGGL(k_amg_bj<T>, g1(V.n), dim3(256), 0, c.stream, V.n, H.nf, W.binv, r, x, x, H.omega,
@@ -1150,10 +1182,7 @@ static void vcycle(Ctx& c, Amg& H, const T* A0, const double* cinv, const T* v, 
       cheb_step<T>(c, V, W, b, x, W.t, 0.0, k.inv_theta, 1);
       cheb_step<T>(c, V, W, b, W.t, x, k.a2, k.b2, 0);
     }
-    by_lanes(V.lanes, [&](auto LN) {
-      constexpr int K = decltype(LN)::value;
-      hipLaunchKernelGGL((k_amg_resid<K, T>), g1(V.n * K), dim3(256), 0, c.stream, V.n, V.rp, V.ci, W.A, b, x, W.r);
-    });
+    level_resid<T>(c, V, W, b, x, W.r);
     const LvView<T> Wc = view<T>(H, l + 1, A0);
     const long nc = H.lv[l + 1]->n;
     by_lanes(V.rlane, [&](auto LN) {

@@ -849,6 +849,14 @@ int mhx_set_option(mhx_ctx* ctx, const char* name, int64_t value) {
     ctx->spmv_lanes = (int)value;
     return MHX_OK;
   }
+  if (k == "krylov.lattice_columns") {
+    if (value != 0 && value != 1) {
+      ctx->err = "krylov.lattice_columns must be 0 or 1";
+      return MHX_ERR_ARG;
+    }
+    ctx->lattice_cols = (int)value;
+    return MHX_OK;
+  }
   if (k == "amg.sweeps") {
     if (value < 1 || value > 16) {
       ctx->err = "amg.sweeps must be 1 .. 16";
@@ -886,6 +894,10 @@ int mhx_get_option(const mhx_ctx* ctx, const char* name, int64_t* value) {
   }
   if (k == "krylov.spmv_lanes") {
     *value = ctx->spmv_lanes;
+    return MHX_OK;
+  }
+  if (k == "krylov.lattice_columns") {
+    *value = ctx->lattice_cols;
     return MHX_OK;
   }
   if (k == "amg.sweeps") {

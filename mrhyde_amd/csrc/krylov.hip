@@ -18,6 +18,7 @@
 #include <type_traits>
 #include <vector>
 
+#include "lattice.hpp"
 #include "mhx_internal.hpp"
 
 namespace mhx {
@@ -29,15 +30,21 @@ constexpr int kScalN = 200;       //                 single reductions
 constexpr int kScalSize = 256;
 constexpr int kMaxRefine = 100;
 
-// y = A (s .* x)  (s == nullptr: plain SpMV).  LANES lanes per row.
-template <int LANES, class TA, class TX>
-__global__ __launch_bounds__(256) void k_spmv(long n, const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
-                                              const TA* __restrict__ A, const TX* __restrict__ s,
-                                              const TX* __restrict__ x, TX* __restrict__ y) {
+// y = A (s .* x)  (s == nullptr: plain SpMV).  LANES lanes per row.  LAT:
+// columns from the closed form of the lattice pattern instead of colidx (the
+// same entries in the same order, so the same sums)
+template <int LANES, bool LAT, class TA, class TX>
+__global__ __launch_bounds__(256) void k_spmv(BoxMesh m, long n, const int32_t* __restrict__ rp,
+                                              const int32_t* __restrict__ colidx, const TA* __restrict__ A,
+                                              const TX* __restrict__ s, const TX* __restrict__ x,
+                                              TX* __restrict__ y) {
   const long row = ((long)blockIdx.x * blockDim.x + threadIdx.x) / LANES;
   const int l = threadIdx.x % LANES;
   if (row >= n) return;
   const int b = rp[row], e = rp[row + 1];
+  std::conditional_t<LAT, LatCols, CsrCols> ci;
+  if constexpr (LAT) ci = LatCols{lat_row(m, row), b};
+  else ci = CsrCols{colidx};
   TX acc = 0;
   // four entries per pass: the value and index loads of all four are issued
   // before the dependent gathers of x, so a lane keeps several misses in flight
@@ -48,13 +55,13 @@ __global__ __launch_bounds__(256) void k_spmv(long n, const int32_t* __restrict_
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       av[u] = A[k + u * LANES];
-      cv[u] = ci[k + u * LANES];
+      cv[u] = ci(k + u * LANES);
     }
 #pragma unroll
     for (int u = 0; u < 4; ++u) acc += (TX)av[u] * (s ? s[cv[u]] * x[cv[u]] : x[cv[u]]);
   }
   for (; k < e; k += LANES) {
-    const int c = ci[k];
+    const int c = ci(k);
     acc += (TX)A[k] * (s ? s[c] * x[c] : x[c]);
   }
 #pragma unroll
@@ -68,13 +75,23 @@ static void spmv_t(Ctx& c, const TA* A, const TX* s, const TX* x, TX* y) {
   int L = c.spmv_lanes;
   long threads = n * L;
   dim3 grid((unsigned)((threads + 255) / 256));
-  switch (L) {
-    case 4: hipLaunchKernelGGL((k_spmv<4, TA, TX>), grid, dim3(256), 0, c.stream, n, c.rowptr.p, c.colidx.p, A, s, x, y); break;
-    case 8: hipLaunchKernelGGL((k_spmv<8, TA, TX>), grid, dim3(256), 0, c.stream, n, c.rowptr.p, c.colidx.p, A, s, x, y); break;
-    case 16: hipLaunchKernelGGL((k_spmv<16, TA, TX>), grid, dim3(256), 0, c.stream, n, c.rowptr.p, c.colidx.p, A, s, x, y); break;
-    case 32: hipLaunchKernelGGL((k_spmv<32, TA, TX>), grid, dim3(256), 0, c.stream, n, c.rowptr.p, c.colidx.p, A, s, x, y); break;
-    default: hipLaunchKernelGGL((k_spmv<64, TA, TX>), grid, dim3(256), 0, c.stream, n, c.rowptr.p, c.colidx.p, A, s, x, y); break;
-  }
+  auto go = [&](auto LN, auto LT) {
+    constexpr int K = decltype(LN)::value;
+    constexpr bool LAT = decltype(LT)::value;
+    hipLaunchKernelGGL((k_spmv<K, LAT, TA, TX>), grid, dim3(256), 0, c.stream, c.mesh, n, c.rowptr.p, c.colidx.p, A,
+                       s, x, y);
+  };
+  auto lanes = [&](auto LT) {
+    switch (L) {
+      case 4: go(std::integral_constant<int, 4>{}, LT); break;
+      case 8: go(std::integral_constant<int, 8>{}, LT); break;
+      case 16: go(std::integral_constant<int, 16>{}, LT); break;
+      case 32: go(std::integral_constant<int, 32>{}, LT); break;
+      default: go(std::integral_constant<int, 64>{}, LT); break;
+    }
+  };
+  if (c.lattice_cols) lanes(std::true_type{});
+  else lanes(std::false_type{});
   MHX_HIP(hipGetLastError());
 }
 
