@@ -198,21 +198,6 @@ __global__ void k_amg_start(long n, double* __restrict__ v) {
   v[i] = __dadd_rn(0.5, __dsub_rn(x, floor(x)));
 }
 
-// w = s * D^-1 A v  (s: scale applied to v, so v need not be normalised in place)
-template <int LANES>
-__global__ __launch_bounds__(256) void k_amg_dinv_spmv(long n, const int32_t* __restrict__ rp,
-                                                       const int32_t* __restrict__ ci, const double* __restrict__ A,
-                                                       const double* __restrict__ dinv, double s,
-                                                       const double* __restrict__ v, double* __restrict__ w) {
-  const long row = ((long)blockIdx.x * blockDim.x + threadIdx.x) / LANES;
-  const int l = threadIdx.x % LANES;
-  if (row >= n) return;
-  double acc = 0.0;
-  for (int k = rp[row] + l; k < rp[row + 1]; k += LANES) acc = fma(A[k], v[ci[k]], acc);
-#pragma unroll
-  for (int off = LANES / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, LANES);
-  if (l == 0) w[row] = s * dinv[row] * acc;
-}
 
 // nrm[(I, f)] = || ns restricted to aggregate I, field f ||
 __global__ void k_amg_nrm(Geo G, const double* __restrict__ ns, double* __restrict__ nrm, long ncrows) {
@@ -672,6 +657,19 @@ __device__ __forceinline__ auto level_cols(const BoxMesh& m, const int32_t* __re
   else return CsrCols{ci};
 }
 
+// w = s * D^-1 A v  (s: scale applied to v, so v need not be normalised in place)
+template <int LANES, bool LAT>
+__global__ __launch_bounds__(256) void k_amg_dinv_spmv(BoxMesh m, long n, const int32_t* __restrict__ rp,
+                                                       const int32_t* __restrict__ ci, const double* __restrict__ A,
+                                                       const double* __restrict__ dinv, double s,
+                                                       const double* __restrict__ v, double* __restrict__ w) {
+  const long row = ((long)blockIdx.x * blockDim.x + threadIdx.x) / LANES;
+  const int l = threadIdx.x % LANES;
+  if (row >= n) return;
+  const double acc = row_dot<LANES, double>(rp, level_cols<LAT>(m, rp, ci, row), A, v, row, l);
+  if (l == 0) w[row] = s * dinv[row] * acc;
+}
+
 template <int LANES, bool LAT, class T>
 __global__ __launch_bounds__(256) void k_amg_cheb(BoxMesh m, long n, const int32_t* __restrict__ rp,
                                                   const int32_t* __restrict__ ci, const T* __restrict__ A,
@@ -834,6 +832,12 @@ static void by_lanes(int lanes, K&& k) {
   }
 }
 
+template <class F>
+static void by_lat(bool lat, F&& f) {
+  if (lat) f(std::true_type{});
+  else f(std::false_type{});
+}
+
 static Geo upload_geo(Ctx& c, AmgLevel& L, const AxisHost* ax, int nf) {
   size_t total = 0;
   for (int d = 0; d < 3; ++d)
@@ -874,8 +878,10 @@ static double power_estimate(Ctx& c, AmgLevel& L, const double* A, double* v, do
   double lam = 1.0;
   for (int it = 0; it < kAmgPowerIters; ++it) {
     by_lanes(L.lanes, [&](auto LN) {
-      hipLaunchKernelGGL((k_amg_dinv_spmv<decltype(LN)::value>), g1(n * LN), dim3(256), 0, c.stream, n, L.rp, L.ci,
-                         A, L.dinv.p, s, v, w);
+      by_lat(L.lat, [&](auto LT) {
+        hipLaunchKernelGGL((k_amg_dinv_spmv<decltype(LN)::value, decltype(LT)::value>), g1(n * LN), dim3(256), 0,
+                           c.stream, c.mesh, n, L.rp, L.ci, A, L.dinv.p, s, v, w);
+      });
     });
     MHX_HIP(hipGetLastError());
     lam = s * dot(c, v, w, n);  // (s v) . (D^-1 A s v)
@@ -1078,12 +1084,6 @@ LvView<float> view<float>(Amg& H, int l, const float* A0) {
   AmgLevel& V = *H.lv[l];
   return {l ? V.f32.A.p : A0, V.f32.dinv.p, V.f32.P.p, V.f32.R.p, V.f32.binv.p, V.f32.x.p, V.f32.t.p, V.f32.b.p,
           V.f32.r.p, V.f32.d.p};
-}
-
-template <class F>
-static void by_lat(bool lat, F&& f) {
-  if (lat) f(std::true_type{});
-  else f(std::false_type{});
 }
 
 template <class T>
