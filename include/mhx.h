@@ -307,6 +307,11 @@ int  mhx_l2_error(mhx_ctx* ctx, const double* u, int32_t n_modes, const double* 
  *   "krylov.spmv_lanes"   lanes that share one CRS row in the sparse
  *                         matrix-vector product: 4, 8, 16, 32 or 64 (default
  *                         chosen from the mean row length at context build).
+ *   "krylov.lattice_columns"  1 (default): sparse products on the
+ *                         context's lattice pattern compute each entry's
+ *                         column from the pattern's closed form instead of
+ *                         reading the column index (same sums); 0: read
+ *                         the column indices.
  *   "amg.smoother"        smoother of the AMG V-cycle: -1 = automatic
  *                         (default: 0 for one unknown per node, 1 for
  *                         coupled fields), 0 = degree-2
