@@ -433,8 +433,8 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=240, help="serving requests (0 = skip)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
-              "(each GPU would own a slab of elements; see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: the multi-GPU form "
+              "(each GPU owning a z-slab of elements) is not built", file=sys.stderr)
         sys.exit(2)
 
     from mrhyde_amd import Mhx
