@@ -240,6 +240,18 @@ def c2_newton_bench(n: int):
     u32, st32 = g.nonlinearSolver(u32, max_iters=5, nl_tol=1e-12, backtracking=False, solver="gmres", restart=50,
                                   tol=1e-10, linear_max_iters=1000, precond="amg-fp32")
     wall32 = time.perf_counter() - t0
+    # hierarchy reuse across the Newton steps (coarse levels of the first step
+    # kept, fine-level smoother data refreshed), FP64 and FP32-refined
+    g.set_option("amg.reuse", 1)
+    reuse = {}
+    for name, prec in (("fp64", "amg"), ("fp32_refined", "amg-fp32")):
+        ur = g.setDirichlet(np.zeros(g.n_dof))
+        t0 = time.perf_counter()
+        ur, str_ = g.nonlinearSolver(ur, max_iters=5, nl_tol=1e-12, backtracking=False, solver="gmres", restart=50,
+                                     tol=1e-10, linear_max_iters=1000, precond=prec)
+        reuse[name] = {"wall_s": round(time.perf_counter() - t0, 3), "krylov_iters": str_["iters"],
+                       "res_norm": str_["res_norm"], "last_linear_rel_res": str_["rel_res"],
+                       "max_abs_diff_vs_fp64": float(np.abs(ur - u).max())}
     g.close()
     ph = {k: round(v[0], 1) for k, v in prof.items()
           if k in ("AssemblyManager::assembleJacRes", "LinearAlgebraInterface::linearSolver",
@@ -253,7 +265,8 @@ def c2_newton_bench(n: int):
                              "refinements": st32["refinements"], "res_norm": st32["res_norm"],
                              "last_linear_rel_res": st32["rel_res"],
                              "max_abs_diff_vs_fp64": float(np.abs(u32 - u).max()),
-                             "max_abs_u": float(np.abs(u).max())}}
+                             "max_abs_u": float(np.abs(u).max())},
+            "amg_reuse": reuse}
 
 
 def c5_requests(count: int, seed: int):

@@ -323,6 +323,11 @@ int  mhx_l2_error(mhx_ctx* ctx, const double* u, int32_t n_modes, const double* 
  *                         "smoother: type" key, CHEBYSHEV or RELAXATION,
  *                         src/interfaces/linear_algebra/
  *                         linearAlgebraInterface_solvers.hpp:225).
+ *   "amg.reuse"           0 (default, the reference's MueLu "reuse: none"):
+ *                         every solve builds its hierarchy; 1: later solves
+ *                         on the context keep the coarse levels of the last
+ *                         full set-up and refresh only the fine level's
+ *                         smoother data from the new operator.
  *   "amg.sweeps"          block-Jacobi smoother sweeps before and after
  *                         the coarse correction, 1 .. 16 (default 3).
  *   "amg.omega_pct"       block-Jacobi damping in percent, 1 .. 200

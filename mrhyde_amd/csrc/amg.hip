@@ -811,6 +811,7 @@ struct Amg {
   double omega = kBjOmega;  // block-Jacobi damping
   int nf = 1;
   int64_t setup_count = 0;
+  int64_t reuse_count = 0;  // set-ups answered by refreshing the fine level only
 };
 
 static int lanes_for(double avg) {
@@ -899,13 +900,43 @@ void amg_setup(Ctx& c, const double* A, bool fp32) {
   Amg& H = *c.amg;
   const int nf = c.nfield;
   MHX_REQUIRE(nf >= 1 && nf <= kAmgMaxFields, MHX_ERR_UNSUPPORTED, "AMG: at most 4 unknowns per node");
+  // automatic choice: Chebyshev on the point diagonal for one unknown per
+  // node, nodal block Jacobi for coupled fields
+  const int smoother = c.amg_smoother >= 0 ? c.amg_smoother : (nf > 1 ? kSmootherBlockJacobi : kSmootherChebyshev);
+  // reuse (MueLu "reuse: full" in kind): keep the coarse levels of the last
+  // set-up and refresh only what the fine-level smoother reads from the new
+  // operator (its diagonal, or its nodal block inverses); the fine operator
+  // itself is the new one, passed to every V-cycle
+  if (c.amg_reuse && H.setup_count > 0 && H.fp32 == fp32 && H.nf == nf && H.smoother == smoother &&
+      !H.lv.empty() && H.lv[0]->n == c.mesh.ndof) {
+    AmgLevel& L0 = *H.lv[0];
+    H.A0 = A;
+    hipLaunchKernelGGL(k_amg_diag, g1(L0.n), dim3(256), 0, c.stream, L0.n, L0.rp, L0.ci, A, L0.dinv.p);
+    if (H.smoother != kSmootherChebyshev && H.lv.size() > 1) {
+      const long nblk = L0.n / nf;
+      switch (nf) {
+        case 1: hipLaunchKernelGGL(k_amg_block_inv<1>, g1(nblk), dim3(256), 0, c.stream, nblk, L0.rp, L0.ci, A, L0.binv.p); break;
+        case 2: hipLaunchKernelGGL(k_amg_block_inv<2>, g1(nblk), dim3(256), 0, c.stream, nblk, L0.rp, L0.ci, A, L0.binv.p); break;
+        case 3: hipLaunchKernelGGL(k_amg_block_inv<3>, g1(nblk), dim3(256), 0, c.stream, nblk, L0.rp, L0.ci, A, L0.binv.p); break;
+        default: hipLaunchKernelGGL(k_amg_block_inv<4>, g1(nblk), dim3(256), 0, c.stream, nblk, L0.rp, L0.ci, A, L0.binv.p); break;
+      }
+    }
+    if (fp32) {
+      hipLaunchKernelGGL((k_amg_cast<double, float>), g1(L0.n), dim3(256), 0, c.stream, L0.n, L0.dinv.p,
+                         L0.f32.dinv.p);
+      if (H.smoother != kSmootherChebyshev && H.lv.size() > 1)
+        hipLaunchKernelGGL((k_amg_cast<double, float>), g1((long)L0.binv.n), dim3(256), 0, c.stream,
+                           (long)L0.binv.n, L0.binv.p, L0.f32.binv.p);
+    }
+    MHX_HIP(hipGetLastError());
+    ++H.reuse_count;
+    return;
+  }
   H.err.alloc(1);
   MHX_HIP(hipMemsetAsync(H.err.p, 0, sizeof(int), c.stream));
   H.A0 = A;
   H.fp32 = fp32;
-  // automatic choice: Chebyshev on the point diagonal for one unknown per
-  // node, nodal block Jacobi for coupled fields
-  H.smoother = c.amg_smoother >= 0 ? c.amg_smoother : (nf > 1 ? kSmootherBlockJacobi : kSmootherChebyshev);
+  H.smoother = smoother;
   H.sweeps = c.amg_sweeps;
   H.omega = c.amg_omega_pct * 0.01;
   H.nf = nf;

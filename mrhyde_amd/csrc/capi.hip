@@ -857,6 +857,14 @@ int mhx_set_option(mhx_ctx* ctx, const char* name, int64_t value) {
     ctx->lattice_cols = (int)value;
     return MHX_OK;
   }
+  if (k == "amg.reuse") {
+    if (value != 0 && value != 1) {
+      ctx->err = "amg.reuse must be 0 or 1";
+      return MHX_ERR_ARG;
+    }
+    ctx->amg_reuse = (int)value;
+    return MHX_OK;
+  }
   if (k == "amg.sweeps") {
     if (value < 1 || value > 16) {
       ctx->err = "amg.sweeps must be 1 .. 16";
@@ -898,6 +906,10 @@ int mhx_get_option(const mhx_ctx* ctx, const char* name, int64_t* value) {
   }
   if (k == "krylov.lattice_columns") {
     *value = ctx->lattice_cols;
+    return MHX_OK;
+  }
+  if (k == "amg.reuse") {
+    *value = ctx->amg_reuse;
     return MHX_OK;
   }
   if (k == "amg.sweeps") {

@@ -213,6 +213,7 @@ struct Ctx {
   int spmv_lanes = 16;
   int lattice_cols = 1;  // sparse products take columns from the lattice closed form (0: from colidx)
   int amg_smoother = -1;  // -1 automatic, 0 Chebyshev(2) on the point diagonal, 1 nodal block Jacobi, 2 block Gauss-Seidel
+  int amg_reuse = 0;       // 1: later AMG set-ups keep the coarse levels and refresh the fine level
   int amg_sweeps = 3;      // block-Jacobi smoother: sweeps before and after the coarse correction
   int amg_omega_pct = 40;  // block-Jacobi smoother: damping in percent
   int nfield = 1;  // unknowns per node (consecutive DOFs): block size of block Jacobi

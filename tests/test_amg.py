@@ -350,3 +350,25 @@ def test_amg_options_are_validated():
             with pytest.raises(MhxError, match=name.split(".")[1]):
                 g.set_option(name, v)
             assert g.get_option(name) == good[-1]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precond", ["amg", "amg-fp32"])
+def test_amg_reuse_newton(precond):
+    """amg.reuse keeps the coarse levels of the first Newton step's hierarchy
+    and refreshes the fine-level smoother data from each new Jacobian: every
+    linear solve still reaches 1e-10 and Newton lands on the same solution."""
+    from mrhyde_amd import Mhx
+
+    p = Problem(dim=3, order=2, quadrature=4, n=(8, 7, 6), diff_u2=1.0, k0=lognormal(336, 0.5, 21),
+                src_elem=np.random.default_rng(22).uniform(0, 10, 336), dbc_sides=1 | 2)
+    g = Mhx(p)
+    sols = []
+    for reuse in (0, 1):
+        g.set_option("amg.reuse", reuse)
+        u = g.setDirichlet(np.zeros(g.n_dof))
+        u, st = g.nonlinearSolver(u, max_iters=8, nl_tol=1e-12, solver="gmres", restart=50, tol=1e-10,
+                                  linear_max_iters=500, precond=precond)
+        assert st["rel_res"] <= 1e-10 and st["res_norm"] <= 1e-12 * st["res_norm0"], (reuse, st)
+        sols.append(u)
+    np.testing.assert_allclose(sols[1], sols[0], rtol=0, atol=1e-9 * np.abs(sols[0]).max())
