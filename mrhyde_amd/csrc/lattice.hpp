@@ -113,9 +113,12 @@ struct LatRow {
 };
 
 __device__ __forceinline__ LatRow lat_row(const BoxMesh& m, long row) {
-  const long node = row / m.nf;
-  const int c[3] = {(int)(node % m.nn[0]), (int)((node / m.nn[0]) % m.nn[1]),
-                    (int)(node / ((long)m.nn[0] * m.nn[1]))};
+  // row and node numbers fit 32 bits (int32 CRS); 32-bit unsigned division
+  // is several times cheaper than the 64-bit one
+  const unsigned node = (unsigned)row / (unsigned)m.nf;
+  const unsigned yz = node / (unsigned)m.nn[0];
+  const unsigned zz = yz / (unsigned)m.nn[1];
+  const int c[3] = {(int)(node - yz * (unsigned)m.nn[0]), (int)(yz - zz * (unsigned)m.nn[1]), (int)zz};
   int lo[3], ext[3];
   for (int a = 0; a < 3; ++a) {
     if (a >= m.dim) {
