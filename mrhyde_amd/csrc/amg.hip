@@ -616,7 +616,7 @@ __global__ void k_amg_bj(long n, int nf, const T* __restrict__ binv, const T* __
                          double omega, int zero_x) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const long node = i / nf;
+  const long node = (long)((unsigned)i / (unsigned)nf);  // rows fit 32 bits
   const int f = (int)(i - node * nf);
   const T* row = binv + node * nf * nf + f * nf;
   const T* rb = r + node * nf;
@@ -728,8 +728,10 @@ __global__ __launch_bounds__(256) void k_amg_gs(long ncol, int nf, int s, int cx
   long node = 0, row = 0;
   double res = 0.0;
   if (active) {
-    const long a = t % cntx, q = t / cntx;
-    node = (cx + s * a) + Nx * ((cy + s * (q % cnty)) + Ny * (cz + s * (q / cnty)));
+    // node numbers fit 32 bits: 32-bit unsigned divisions
+    const unsigned tu = (unsigned)t, q = tu / (unsigned)cntx, qz = q / (unsigned)cnty;
+    const long a = (long)(tu - q * (unsigned)cntx), qy = (long)(q - qz * (unsigned)cnty);
+    node = (cx + s * a) + Nx * ((cy + s * qy) + Ny * (cz + s * (long)qz));
     row = node * nf + f;
     res = (double)b[row] - row_dot<LANES, T>(rp, CsrCols{ci}, A, x, row, l);
   }
