@@ -324,12 +324,16 @@ def test_ns_newton_with_block_jacobi_amg():
         u_ref = u_ref + spla.spsolve(sp.csr_matrix((j, ci, rp), shape=(o.n_dof, o.n_dof)).tocsc(), r)
     g = Mhx(p)
     g.set_option("amg.smoother", 1)
-    print("[amg-newton] 10^3 cavity", flush=True)
-    u, st = g.nonlinearSolver(u0.copy(), max_iters=10, nl_tol=1e-12, solver="gmres", restart=50, tol=1e-10,
-                              linear_max_iters=500, precond="amg")
-    assert st["rel_res"] <= 1e-10 and st["res_norm"] <= 1e-12 * st["res_norm0"], st
-    assert st["iters"] <= 50 * st["nl_iters"], st
-    np.testing.assert_allclose(u, u_ref, rtol=0, atol=1e-9 * np.abs(u_ref).max())
+    # also with the hierarchy kept across the Newton steps (fine-level block
+    # inverses refreshed) and with the FP32 cycle in FP64 refinement
+    for reuse, prec in ((0, "amg"), (1, "amg"), (1, "amg-fp32")):
+        g.set_option("amg.reuse", reuse)
+        print(f"[amg-newton] 10^3 cavity reuse={reuse} {prec}", flush=True)
+        u, st = g.nonlinearSolver(u0.copy(), max_iters=10, nl_tol=1e-12, solver="gmres", restart=50, tol=1e-10,
+                                  linear_max_iters=500, precond=prec)
+        assert st["rel_res"] <= 1e-10 and st["res_norm"] <= 1e-12 * st["res_norm0"], (reuse, prec, st)
+        assert st["iters"] <= 60 * st["nl_iters"], (reuse, prec, st)
+        np.testing.assert_allclose(u, u_ref, rtol=0, atol=1e-9 * np.abs(u_ref).max())
 
 
 @pytest.mark.gpu
