@@ -905,6 +905,11 @@ static void launch_colors(Ctx& c, AsmArgs& a, bool jac) {
   const int ncol = 1 << DIM;
   // the transient variant is built for element colouring only
   const int mode = a.hist ? (int)kElementColouring : c.scatter_mode;
+  // x-segments are verified with at least 8 elements per workgroup; a 96^3
+  // Q2 build with 4 per workgroup disagreed with element colouring beyond
+  // round-off, so smaller segments are refused rather than run
+  MHX_REQUIRE(mode != kSegmentMerge || C::GPB >= 8, MHX_ERR_UNSUPPORTED,
+              "assembly.colouring 2 (x-segments) needs at least 8 elements per workgroup for this element order");
   a.gpb = C::GPB;
   a.blocked = mode;
   a.nsub = mode == kBlockColouring ? ncol : 1;
