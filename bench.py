@@ -40,13 +40,9 @@ PMC_TRAFFIC = {(96, 0): {"fetch_kib": 529028.5, "write_kib": 1032647.0, "fetch_s
 
 
 def c2_problem(n: int):
-    from mrhyde_amd.abi import MHX_RNG_LOGNORMAL, MHX_RNG_UNIFORM, Problem, rng_fill
+    from mrhyde_amd.workloads import thermal_q2
 
-    ne = n ** 3
-    k0 = rng_fill(2024, ne, MHX_RNG_LOGNORMAL, 0.0, 0.5)
-    f = rng_fill(2025, ne, MHX_RNG_UNIFORM, 0.0, 1.0)
-    return Problem(dim=3, order=2, quadrature=4, n=(n, n, n), k0=k0, src_elem=f, diff_u2=1.0,
-                   dbc_sides=1 | 2)
+    return thermal_q2(n)
 
 
 def b_asm(ne: int, n_e: int, n_c: int, n_vert: int, dim: int) -> float:
@@ -147,13 +143,12 @@ def ns_bench(n: int, steps: int, newton: bool = True):
     by nodal block Jacobi; then Newton with GMRES(50) + that AMG cycle to
     1e-10 per step, FP32 hierarchy and inner solve in FP64 refinement,
     against the all-FP64 Newton (difference stated)."""
-    from mrhyde_amd import Mhx, rng
-    from mrhyde_amd.abi import MHX_BUF_U, MHX_PHYS_NAVIER_STOKES, Problem
+    from mrhyde_amd import Mhx
+    from mrhyde_amd.abi import MHX_BUF_U
+    from mrhyde_amd.workloads import cavity, cavity_state
 
-    p = Problem(dim=3, order=1, quadrature=2, n=(n, n, n), physics=MHX_PHYS_NAVIER_STOKES, mu=0.01, supg=True,
-                pspg=True, dbc_sides=1 | 2 | 4 | 16 | 32, lid_sides=8, lid_u=(1.0, 0.0, 0.0), pin_pressure=True)
-    g = Mhx(p)
-    u0 = g.setDirichlet(1e-3 * rng(2027, g.n_dof, 2, 0.0, 1.0))
+    g = Mhx(cavity(n))
+    u0 = g.setDirichlet(cavity_state(g.n_dof))
     g.state_upload(MHX_BUF_U, u0)
     for _ in range(2):
         g.assemble_resident()
@@ -270,26 +265,9 @@ def c2_newton_bench(n: int):
 
 
 def c5_requests(count: int, seed: int):
-    """Thermal solve requests: with probability 1/2 a 2-D Q1 mesh of n^2
-    elements, n log-uniform on [32, 256], else a 3-D mesh of n^3 elements, n
-    log-uniform on [16, 64], Q1 or Q2 with probability 1/2; k = exp(0.5 z) per
-    element; source sum_{m=1..M} a_m prod_d sin(w_md pi x_d), M ~ U{1..4},
-    a_m ~ U[0,1], w ~ U{1..4}; T = 0 on the boundary."""
-    from mrhyde_amd.abi import Problem
+    from mrhyde_amd.workloads import serve_requests
 
-    rng = np.random.default_rng(seed)
-    out = []
-    for _ in range(count):
-        if rng.random() < 0.5:
-            dim, order, n = 2, 1, int(round(np.exp(rng.uniform(np.log(32), np.log(256)))))
-        else:
-            dim, order, n = 3, int(rng.integers(1, 3)), int(round(np.exp(rng.uniform(np.log(16), np.log(64)))))
-        ne = n ** dim
-        modes = [(float(rng.uniform(0, 1)), [np.pi * int(rng.integers(1, 5)) for _ in range(dim)])
-                 for _ in range(int(rng.integers(1, 5)))]
-        out.append(Problem(dim=dim, order=order, quadrature=2 * order, n=(n,) * dim,
-                           k0=np.exp(0.5 * rng.standard_normal(ne)), modes=modes))
-    return out
+    return serve_requests(count, seed)
 
 
 def serving_bench(n_req: int = 240, rate: float = 200.0, seed: int = 5, per_call_sample: int = 12):

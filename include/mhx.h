@@ -327,7 +327,13 @@ int  mhx_l2_error(mhx_ctx* ctx, const double* u, int32_t n_modes, const double* 
  *                         every solve builds its hierarchy; 1: later solves
  *                         on the context keep the coarse levels of the last
  *                         full set-up and refresh only the fine level's
- *                         smoother data from the new operator.
+ *                         smoother data from the new operator (diagonal or
+ *                         nodal block inverses; "amg.sweeps" and
+ *                         "amg.omega_pct" take their current values).  The
+ *                         Chebyshev eigenvalue bound of every level stays the
+ *                         one estimated at that full set-up; a change of
+ *                         precision, smoother or unknowns per node forces a
+ *                         full set-up.
  *   "amg.sweeps"          block-Jacobi smoother sweeps before and after
  *                         the coarse correction, 1 .. 16 (default 3).
  *   "amg.omega_pct"       block-Jacobi damping in percent, 1 .. 200

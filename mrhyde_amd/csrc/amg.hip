@@ -918,6 +918,9 @@ void amg_setup(Ctx& c, const double* A, bool fp32) {
       !H.lv.empty() && H.lv[0]->n == c.mesh.ndof) {
     AmgLevel& L0 = *H.lv[0];
     H.A0 = A;
+    // cycle parameters are read per V-cycle: take the current options
+    H.sweeps = c.amg_sweeps;
+    H.omega = c.amg_omega_pct * 0.01;
     hipLaunchKernelGGL(k_amg_diag, g1(L0.n), dim3(256), 0, c.stream, L0.n, L0.rp, L0.ci, A, L0.dinv.p);
     if (H.smoother != kSmootherChebyshev && H.lv.size() > 1) {
       const long nblk = L0.n / nf;

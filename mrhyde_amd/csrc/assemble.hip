@@ -148,6 +148,7 @@ struct Cfg {
   static constexpr int BLOCK = 256;
   static constexpr int GPB = BLOCK / G;
   static_assert(G <= 64, "element needs more than one wavefront");
+  static_assert(BLOCK <= 256 && BLOCK % 64 == 0, "k_thermal is built with __launch_bounds__(256)");
 };
 
 // ---------------------------------------------------------------------------
@@ -623,8 +624,8 @@ __global__ __launch_bounds__(256) void k_thermal(AsmArgs a) {
     const int Xend = P * s_act;    // last plane holding this segment's entries
     const int yb = P * ec[1], zb = P * ec[2];
     if constexpr (JAC)
-      for (int k = threadIdx.x; k < SG::kSize; k += 256) s_seg[k] = 0.0;
-    for (int k = threadIdx.x; k < SG::kRows; k += 256) {
+      for (int k = threadIdx.x; k < SG::kSize; k += C::BLOCK) s_seg[k] = 0.0;
+    for (int k = threadIdx.x; k < SG::kRows; k += C::BLOCK) {
       s_sres[k] = 0.0;
       const int X = k / SG::RP, r = k % SG::RP;
       const int Xg = X0g + X;
@@ -666,7 +667,7 @@ __global__ __launch_bounds__(256) void k_thermal(AsmArgs a) {
     }
     // residual rows
     if (a.res) {
-      for (int k = threadIdx.x; k < SG::kRows; k += 256) {
+      for (int k = threadIdx.x; k < SG::kRows; k += C::BLOCK) {
         const int X = k / SG::RP, r = k % SG::RP;
         const int Xg = X0g + X;
         if (X > Xend || Xg >= a.m.nn[0]) continue;
@@ -690,7 +691,7 @@ __global__ __launch_bounds__(256) void k_thermal(AsmArgs a) {
     if constexpr (JAC) {
       if (a.jac) {
         constexpr int CH = 8;
-        for (int base = threadIdx.x; base < SG::kSize; base += 256 * CH) {
+        for (int base = threadIdx.x; base < SG::kSize; base += C::BLOCK * CH) {
           int sl[CH], mode[CH];
           double val[CH], old[CH];
 #pragma unroll
@@ -698,7 +699,7 @@ __global__ __launch_bounds__(256) void k_thermal(AsmArgs a) {
             mode[c] = 0;
             sl[c] = 0;
             val[c] = 0.0;
-            const int eix = base + c * 256;
+            const int eix = base + c * C::BLOCK;
             if (eix >= SG::kSize) continue;
             // decode (plane, row, column block, x offset); every divisor is a
             // compile-time constant (vertex and interior planes separately)
@@ -905,11 +906,9 @@ static void launch_colors(Ctx& c, AsmArgs& a, bool jac) {
   const int ncol = 1 << DIM;
   // the transient variant is built for element colouring only
   const int mode = a.hist ? (int)kElementColouring : c.scatter_mode;
-  // x-segments are verified with at least 8 elements per workgroup; a 96^3
-  // Q2 build with 4 per workgroup disagreed with element colouring beyond
-  // round-off, so smaller segments are refused rather than run
-  MHX_REQUIRE(mode != kSegmentMerge || C::GPB >= 8, MHX_ERR_UNSUPPORTED,
-              "assembly.colouring 2 (x-segments) needs at least 8 elements per workgroup for this element order");
+  // the segment schedule needs two elements per workgroup (even and odd
+  // phases); every built order has at least four
+  static_assert(C::GPB >= 2, "x-segments need two elements per workgroup");
   a.gpb = C::GPB;
   a.blocked = mode;
   a.nsub = mode == kBlockColouring ? ncol : 1;

@@ -284,7 +284,15 @@ int mhx_server_info(mhx_server* s, int64_t* out8) {
   return MHX_OK;
 }
 
-const char* mhx_server_last_error(mhx_server* s) { return s ? s->err.c_str() : ""; }
+// Copied under the server lock (workers assign s->err from their threads)
+// into a buffer of the calling thread; valid until that thread's next call.
+const char* mhx_server_last_error(mhx_server* s) {
+  thread_local std::string buf;
+  if (!s) return "";
+  std::lock_guard<std::mutex> lk(s->mu);
+  buf = s->err;
+  return buf.c_str();
+}
 
 void mhx_server_destroy(mhx_server* s) {
   if (!s) return;

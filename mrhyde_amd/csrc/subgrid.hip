@@ -411,19 +411,38 @@ static void enqueue(Batch& b, int wg) {
   MHX_HIP(hipGetLastError());
 }
 
+// a pair of timing events destroyed on every exit path, including the
+// MHX_HIP throws between creation and the end of the timed region
+struct EventPair {
+  hipEvent_t a = nullptr, b = nullptr;
+  EventPair() {
+    MHX_HIP(hipEventCreate(&a));
+    hipError_t e = hipEventCreate(&b);
+    if (e != hipSuccess) {
+      (void)hipEventDestroy(a);
+      throw Error{MHX_ERR_DEVICE, std::string("hipEventCreate: ") + hipGetErrorString(e)};
+    }
+  }
+  EventPair(const EventPair&) = delete;
+  EventPair& operator=(const EventPair&) = delete;
+  ~EventPair() {
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+  }
+  float elapsed_ms() const {
+    float ms = 0.f;
+    MHX_HIP(hipEventElapsedTime(&ms, a, b));
+    return ms;
+  }
+};
+
 static double timed_run(Batch& b, int wg) {
-  hipEvent_t e0, e1;
-  MHX_HIP(hipEventCreate(&e0));
-  MHX_HIP(hipEventCreate(&e1));
-  MHX_HIP(hipEventRecord(e0, b.stream));
+  EventPair ev;
+  MHX_HIP(hipEventRecord(ev.a, b.stream));
   enqueue(b, wg);
-  MHX_HIP(hipEventRecord(e1, b.stream));
-  MHX_HIP(hipEventSynchronize(e1));
-  float ms = 0.f;
-  MHX_HIP(hipEventElapsedTime(&ms, e0, e1));
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  return ms;
+  MHX_HIP(hipEventRecord(ev.b, b.stream));
+  MHX_HIP(hipEventSynchronize(ev.b));
+  return ev.elapsed_ms();
 }
 
 static void setup_grid(Batch& b) {
@@ -525,9 +544,7 @@ int mhx_batch_solve(mhx_batch* b, int32_t flags) {
   (void)flags;
   try {
     if (!b) return MHX_ERR_ARG;
-    hipEvent_t e0, e1;
-    MHX_HIP(hipEventCreate(&e0));
-    MHX_HIP(hipEventCreate(&e1));
+    EventPair ev;
     if (b->desc.use_graph) {
       if (!b->exec || b->graph_wg != b->wg) {
         if (b->exec) (void)hipGraphExecDestroy(b->exec);
@@ -540,19 +557,15 @@ int mhx_batch_solve(mhx_batch* b, int32_t flags) {
         MHX_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
         b->graph_wg = b->wg;
       }
-      MHX_HIP(hipEventRecord(e0, b->stream));
+      MHX_HIP(hipEventRecord(ev.a, b->stream));
       MHX_HIP(hipGraphLaunch(b->exec, b->stream));
     } else {
-      MHX_HIP(hipEventRecord(e0, b->stream));
+      MHX_HIP(hipEventRecord(ev.a, b->stream));
       enqueue(*b, b->wg);
     }
-    MHX_HIP(hipEventRecord(e1, b->stream));
-    MHX_HIP(hipEventSynchronize(e1));
-    float ms = 0.f;
-    MHX_HIP(hipEventElapsedTime(&ms, e0, e1));
-    b->last_ms = ms;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    MHX_HIP(hipEventRecord(ev.b, b->stream));
+    MHX_HIP(hipEventSynchronize(ev.b));
+    b->last_ms = ev.elapsed_ms();
     return MHX_OK;
   } catch (const Error& e) {
     if (b) b->err = e.msg;

@@ -313,6 +313,23 @@ def cheb_coef(lam):
     return 1.0 / theta, rho1 * rho0, 2.0 * rho1 / delta
 
 
+def chebyshev(A, dinv, lam, b, x=None):
+    """Two Chebyshev steps on D^-1 A (Ifpack2's three-term recurrence with the
+    coefficients of cheb_coef), from x, or from zero when x is None (the
+    pre-smoother, whose first residual is b).  The error x* - x is multiplied
+    by the degree-2 residual polynomial T_2((theta - t) / delta) / T_2(theta / delta)
+    of t = D^-1 A."""
+    it, a2, b2 = cheb_coef(lam)
+    if x is None:
+        d = it * dinv * b
+        x = d.copy()
+    else:
+        d = it * dinv * (b - A @ x)
+        x = x + d
+    d = a2 * d + b2 * dinv * (b - A @ x)
+    return x + d
+
+
 BJ_OMEGA = 0.4
 BJ_SWEEPS = 3
 
@@ -377,11 +394,7 @@ def vcycle(levels, b, smoother="chebyshev", nf=1):
         A, dinv = L["A"], L["dinv"]
         b = np.asarray(b, dtype=np.float64)
         if smoother == "chebyshev":
-            it, a2, b2 = cheb_coef(L["lam"])
-            d = it * dinv * b
-            x = d.copy()
-            d = a2 * d + b2 * dinv * (b - A @ x)
-            x = x + d
+            x = chebyshev(A, dinv, L["lam"], b)
         else:
             if "binv" not in L:
                 L["binv"] = block_inverses(A, nf)
@@ -394,10 +407,7 @@ def vcycle(levels, b, smoother="chebyshev", nf=1):
         xc = rec(l + 1, L["R"] @ (b - A @ x))
         x = x + L["P"] @ xc
         if smoother == "chebyshev":
-            d = it * dinv * (b - A @ x)
-            x = x + d
-            d = a2 * d + b2 * dinv * (b - A @ x)
-            return x + d
+            return chebyshev(A, dinv, L["lam"], b, x)
         if smoother == "block_jacobi":
             for _ in range(BJ_SWEEPS):
                 x = x + BJ_OMEGA * block_apply(L["binv"], b - A @ x, nf)

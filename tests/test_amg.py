@@ -123,6 +123,57 @@ def test_oracle_vcycle_pcg(name):
     assert it[0] <= 30 and it[0] * 3 < it[1], it
 
 
+@pytest.mark.parametrize("start", ["zero", "warm"])
+@pytest.mark.parametrize("lam_scale", [1.0, 0.8])
+def test_oracle_chebyshev_is_degree2_residual_polynomial(start, lam_scale):
+    """The Chebyshev(2) smoother (Ifpack2 defaults that the reference's MueLu
+    list leaves unset, linearAlgebraInterface_solvers.hpp:208-253: eigenvalue
+    ratio 7, boost 1.1) multiplies the error along each eigenvector of D^-1 A
+    (eigenvalue t) by p(t) = T_2((theta - t) / delta) / T_2(theta / delta) on
+    [lmin, lmax] = [1.1 lam / 7, 1.1 lam], T_2(s) = 2 s^2 - 1; from zero (the
+    pre-smoother) and from a nonzero iterate (the post-smoother).  lam below
+    the spectral radius (0.8) leaves eigenvalues above lmax, where |p|
+    exceeds the minimax bound 1 / T_2(theta / delta)."""
+    from oracle import Oracle
+    from oracle import amg as oa
+
+    from conftest import to_scipy
+
+    p = Problem(dim=2, order=1, quadrature=2, n=(7, 6), k0=lognormal(42, 0.7, 5), src_elem=np.ones(42),
+                dbc_sides=1 | 4, perturb=0.2, perturb_seed=6)
+    o = Oracle(p)
+    _, j = o.assembleJacRes(o.setDirichlet(np.zeros(o.n_dof)))
+    rp, ci = o.crs_pattern()
+    mask, _ = o.dirichlet()
+    free = mask == 0
+    A = to_scipy(rp, ci, j)[free][:, free].toarray()  # the SPD free block
+    d = np.diag(A)
+    s = 1.0 / np.sqrt(d)
+    t, W = np.linalg.eigh(s[:, None] * A * s[None, :])  # D^-1/2 A D^-1/2 = W diag(t) W^T
+    V = s[:, None] * W  # eigenvectors of D^-1 A
+    lam = lam_scale * t.max()
+    lmax = 1.1 * lam
+    lmin = lmax / 7.0
+    theta, delta = 0.5 * (lmax + lmin), 0.5 * (lmax - lmin)
+    T2 = lambda z: 2 * z * z - 1
+    poly = T2((theta - t) / delta) / T2(theta / delta)
+    As = sp.csr_matrix(A)
+    rng = np.random.default_rng(7)
+    xs = V @ rng.standard_normal(len(t))  # exact solution with every eigencomponent
+    b = A @ xs
+    x0 = None if start == "zero" else rng.standard_normal(len(t))
+    e0 = xs if x0 is None else xs - x0
+    x1 = oa.chebyshev(As, 1.0 / d, lam, b, x0)
+    c0 = np.linalg.solve(V, e0)
+    c1 = np.linalg.solve(V, xs - x1)
+    np.testing.assert_allclose(c1, poly * c0, rtol=0, atol=1e-10 * np.abs(c0).max())
+    bound = 1.0 / T2(theta / delta)  # the minimax value on [lmin, lmax]
+    inside = (t >= lmin) & (t <= lmax)
+    assert np.abs(poly[inside]).max() <= bound + 1e-12
+    if lam_scale < 1.0:
+        assert t.max() > lmax and np.abs(poly[t > lmax]).max() > bound
+
+
 # ---- GPU: device hierarchy and solves ---------------------------------------
 def _close(a, b, what, rel=1e-12):
     a, b = a.tocsr(), b.tocsr()

@@ -31,6 +31,13 @@ def make_configs():
     cfg["2d_q3_ragged_1x7"] = Problem(dim=2, order=3, quadrature=6, n=(1, 7), k0=lognormal(7, 1.0, 3),
                                       src_elem=np.linspace(-1, 1, 7), dbc_sides=4)
     cfg["2d_q1_underintegrated"] = Problem(dim=2, order=1, quadrature=0, n=(5, 6), src_elem=np.ones(30))
+    # the highest built 2-D orders: four elements per workgroup, so the x-segment
+    # schedule runs segments of 4, 4 and 1 (Q5) and 4 and 2 (Q6) elements
+    cfg["2d_q5_quad10_perturbed"] = Problem(dim=2, order=5, quadrature=10, n=(9, 3), diff_u2=0.5,
+                                            k0=lognormal(27, 0.5, 13), modes=[(5.0, (PI, 2 * PI))], perturb=0.15,
+                                            perturb_seed=14, dbc_sides=1 | 4)
+    cfg["2d_q6_quad12"] = Problem(dim=2, order=6, quadrature=12, n=(6, 2), diff_u2=1.0, k0=lognormal(12, 0.5, 15),
+                                  src_elem=np.linspace(0, 1, 12), dbc_sides=1 | 8)
     cfg["3d_q1_lognormal"] = Problem(dim=3, order=1, quadrature=2, n=(6, 5, 4), k0=lognormal(120, 1.0, 4),
                                      src_elem=np.random.default_rng(5).uniform(0, 1, 120))
     cfg["3d_q1_single_element"] = Problem(dim=3, order=1, quadrature=2, n=(1, 1, 1), dbc_sides=0,
