@@ -234,6 +234,27 @@ int  mhx_spmv(mhx_ctx* ctx, const double* jac_vals, const double* x, double* y, 
 int  mhx_linear_solve(mhx_ctx* ctx, const double* jac_vals, const double* r, double* x,
                       const mhx_krylov_opts* opts, mhx_solve_stats* stats);
 
+/* Allocation-free variant of mhx_linear_solve.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  It runs the same kernels in the same order as
+ * mhx_linear_solve with solver MHX_KRYLOV_CG and precond MHX_PREC_NONE or
+ * MHX_PREC_JACOBI (FP64), so x, the iteration count and rel_res are the
+ * same bits; the Krylov vectors, the Jacobi scaling and the reduction
+ * partials live in one file-scope device array per GPU of
+ * 5 * MHX_NOALLOC_MAX_DOF + 1280 doubles (40 MiB, held by every GPU the
+ * library runs on until the process ends) instead of context buffers.
+ * It calls no hipMalloc/hipFree and records no phase timers.
+ *   - n_dof <= MHX_NOALLOC_MAX_DOF, else MHX_ERR_ARG;
+ *   - opts->flags must carry MHX_DEVICE_PTRS (r, x and jac_vals are device
+ *     pointers), else MHX_ERR_ARG; jac_vals == NULL: the context's Jacobian;
+ *   - any other solver or preconditioner: MHX_ERR_UNSUPPORTED.
+ * Every call on a GPU shares that GPU's array and leaves its contents
+ * behind: calls that overlap on one GPU (any contexts, any threads) race. */
+#define MHX_NOALLOC_MAX_DOF 1048576
+int  mhx_linear_solve_noalloc(mhx_ctx* ctx, const double* jac_vals, const double* r, double* x,
+                              const mhx_krylov_opts* opts, mhx_solve_stats* stats);
+
 /* MatrixMarket files.  Replaces LinearAlgebraInterface::writeToFile
  * (src/interfaces/linear_algebra/linearAlgebraInterface_util.hpp:316-331,
  * Tpetra::MatrixMarket::Writer::writeSparseFile / writeDenseFile): J as

@@ -32,6 +32,7 @@ def _load() -> C.CDLL:
         "mhx_assemble": (C.c_int, [vp, vp, i32, vp, vp]),
         "mhx_spmv": (C.c_int, [vp, vp, vp, vp, i32]),
         "mhx_linear_solve": (C.c_int, [vp, vp, vp, vp, C.POINTER(KrylovOpts), C.POINTER(SolveStats)]),
+        "mhx_linear_solve_noalloc": (C.c_int, [vp, vp, vp, vp, C.POINTER(KrylovOpts), C.POINTER(SolveStats)]),
         "mhx_nonlinear_solve": (C.c_int, [vp, vp, C.POINTER(NewtonOpts), C.POINTER(SolveStats)]),
         "mhx_l2_error": (C.c_int, [vp, vp, i32, vp, vp, i32, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
         "mhx_profile_enable": (C.c_int, [vp, i32]),
@@ -88,5 +89,5 @@ EXPORTED = [
     "mhx_footprint", "mhx_ctx_update", "mhx_server_create", "mhx_server_submit", "mhx_server_poll",
     "mhx_server_info", "mhx_server_last_error", "mhx_server_destroy", "mhx_amg_setup", "mhx_amg_level",
     "mhx_set_time_state", "mhx_write_matrix_market", "mhx_write_vector_market", "mhx_read_matrix_market",
-    "mhx_read_vector_market",
+    "mhx_read_vector_market", "mhx_linear_solve_noalloc",
 ]

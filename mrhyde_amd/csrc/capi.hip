@@ -519,6 +519,25 @@ int mhx_linear_solve(mhx_ctx* ctx, const double* jac_vals, const double* r, doub
   MHX_GUARD_END(ctx)
 }
 
+int mhx_linear_solve_noalloc(mhx_ctx* ctx, const double* jac_vals, const double* r, double* x,
+                             const mhx_krylov_opts* opts, mhx_solve_stats* stats) {
+  MHX_GUARD_BEGIN
+  MHX_REQUIRE(ctx && r && x && opts, MHX_ERR_ARG, "null argument");
+  MHX_REQUIRE((opts->flags & MHX_DEVICE_PTRS) != 0, MHX_ERR_ARG,
+              "allocation-free solve: r, x and jac_vals must be device pointers (MHX_DEVICE_PTRS)");
+  const double* A = jac_vals;
+  if (!A) {
+    MHX_REQUIRE(ctx->jac_valid && ctx->jac.p, MHX_ERR_STATE, "no Jacobian held by the context");
+    A = ctx->jac.p;
+  }
+  if (stats) std::memset(stats, 0, sizeof(*stats));
+  int s = krylov_solve_noalloc(*ctx, A, r, x, *opts, stats);
+  MHX_HIP(hipStreamSynchronize(ctx->stream));
+  if (stats) stats->status = s;
+  return s;
+  MHX_GUARD_END(ctx)
+}
+
 int mhx_set_time_state(mhx_ctx* ctx, double dt, int32_t bdf_order, const double* u_prev, int32_t flags) {
   MHX_GUARD_BEGIN
   MHX_REQUIRE(ctx, MHX_ERR_ARG, "null argument");

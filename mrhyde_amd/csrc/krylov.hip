@@ -15,6 +15,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <type_traits>
 #include <vector>
 
@@ -173,22 +174,30 @@ static void ensure_red(Ctx& c, int K) {
   if (c.kw.scal.n < (size_t)kScalSize) c.kw.scal.alloc(kScalSize);
 }
 
-static double fetch(Ctx& c, int nb, int mode = 0) {
-  hipLaunchKernelGGL(k_final, dim3(1), dim3(256), 0, c.stream, c.kw.part.p, nb, 1, mode, c.kw.scal.p + kScalN);
+// the nb partials at part reduced into *slot (device) and returned to the host
+static double fetch_to(Ctx& c, const double* part, double* slot, int nb, int mode) {
+  hipLaunchKernelGGL(k_final, dim3(1), dim3(256), 0, c.stream, part, nb, 1, mode, slot);
   MHX_HIP(hipGetLastError());
   double h;
-  MHX_HIP(hipMemcpyAsync(&h, c.kw.scal.p + kScalN, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  MHX_HIP(hipMemcpyAsync(&h, slot, sizeof(double), hipMemcpyDeviceToHost, c.stream));
   MHX_HIP(hipStreamSynchronize(c.stream));
   return h;
+}
+
+static double fetch(Ctx& c, int nb, int mode = 0) { return fetch_to(c, c.kw.part.p, c.kw.scal.p + kScalN, nb, mode); }
+
+template <class T>
+static double reduce_to(Ctx& c, long n, const T* a, const T* b, int mode, double* part, double* slot) {
+  const int nb = red_blocks(n);
+  hipLaunchKernelGGL(k_dot<T>, dim3(nb), dim3(256), 0, c.stream, n, a, b, part, mode);
+  MHX_HIP(hipGetLastError());
+  return fetch_to(c, part, slot, nb, mode);
 }
 
 template <class T>
 static double reduce_to_host(Ctx& c, long n, const T* a, const T* b, int mode) {
   ensure_red(c, 1);
-  const int nb = red_blocks(n);
-  hipLaunchKernelGGL(k_dot<T>, dim3(nb), dim3(256), 0, c.stream, n, a, b, c.kw.part.p, mode);
-  MHX_HIP(hipGetLastError());
-  return fetch(c, nb, mode);
+  return reduce_to<T>(c, n, a, b, mode, c.kw.part.p, c.kw.scal.p + kScalN);
 }
 
 template <class T>
@@ -489,53 +498,59 @@ __global__ __launch_bounds__(256) void k_cg_update(long n, const double* __restr
   block_store1(v, part);
 }
 
+// Device storage of one CG solve: the four Krylov vectors, the reduction
+// partials, the CG scalars and one reduction slot.
 template <class T>
-static int cg_core(Ctx& c, const T* A, const Prec<T>& P, const T* b, T* x, double tol, int maxit, double* relres) {
+struct CgBufs {
+  T *r, *z, *p, *q;
+  double *part, *cgs, *slot;
+};
+
+template <class T>
+static int cg_run(Ctx& c, const T* A, const Prec<T>& P, const T* b, T* x, double tol, int maxit, double* relres,
+                  const CgBufs<T>& v, bool timers) {
   const long n = c.mesh.ndof;
-  KrylovVecs<T>& v = vecs<T>(c);
-  v.r.alloc(n); v.z.alloc(n); v.p.alloc(n); v.q.alloc(n);
-  ensure_red(c, 1);
-  const int W = n < 2000000 ? 16 : 4;  // iterations per host check
-  if (c.kw.cgs.n < (size_t)(kCgHist + W)) c.kw.cgs.alloc(kCgHist + 16);
-  double* cgs = c.kw.cgs.p;
+  constexpr int kMaxW = 16;
+  const int W = n < 2000000 ? kMaxW : 4;  // iterations per host check
+  double* cgs = v.cgs;
   const int nb = red_blocks(n);
-  const double bn = norm2_t<T>(c, b, n);
+  auto nrm = [&](const T* a) { return std::sqrt(reduce_to<T>(c, n, a, a, 0, v.part, v.slot)); };
+  const double bn = nrm(b);
   if (bn == 0.0) {
     MHX_HIP(hipMemsetAsync(x, 0, n * sizeof(T), c.stream));
     *relres = 0.0;
     return 0;
   }
-  spmv_t<T, T>(c, A, nullptr, x, v.q.p);
-  hipLaunchKernelGGL(k_resid<T>, grid1(n), dim3(256), 0, c.stream, n, b, v.q.p, v.r.p);
-  double rn = norm2_t<T>(c, v.r.p, n);
+  spmv_t<T, T>(c, A, nullptr, x, v.q);
+  hipLaunchKernelGGL(k_resid<T>, grid1(n), dim3(256), 0, c.stream, n, b, v.q, v.r);
+  double rn = nrm(v.r);
   if (!std::isfinite(rn)) throw Error{MHX_ERR_NONFINITE, "CG: non-finite residual"};
   int it = 0;
-  std::vector<double> hist(W);
+  double hist[kMaxW];
+  Profiler off;
   while (it < maxit && rn / bn > tol) {
     const int w = std::min(W, maxit - it);
     {
-      ScopedPhase ph(c.prof, "Krylov::cg_window", c.stream);
+      ScopedPhase ph(timers ? c.prof : off, "Krylov::cg_window", c.stream);
       for (int k = 0; k < w; ++k) {
         const int cur = (it + k) & 1;
         if (P.kind == MHX_PREC_BLOCK_JACOBI || P.kind == MHX_PREC_AMG) {
-          prec_apply<T>(c, P, v.r.p, v.z.p);
-          hipLaunchKernelGGL(k_dot<T>, dim3(nb), dim3(256), 0, c.stream, n, v.r.p, v.z.p, c.kw.part.p, 0);
+          prec_apply<T>(c, P, v.r, v.z);
+          hipLaunchKernelGGL(k_dot<T>, dim3(nb), dim3(256), 0, c.stream, n, v.r, v.z, v.part, 0);
         } else {
           hipLaunchKernelGGL(k_prec_dot<T>, dim3(nb), dim3(256), 0, c.stream, n,
-                             P.kind == MHX_PREC_JACOBI ? P.d : (const T*)nullptr, v.r.p, v.z.p, c.kw.part.p);
+                             P.kind == MHX_PREC_JACOBI ? P.d : (const T*)nullptr, v.r, v.z, v.part);
         }
-        hipLaunchKernelGGL(k_final, dim3(1), dim3(256), 0, c.stream, c.kw.part.p, nb, 1, 0, cgs + kCgRho + cur);
-        hipLaunchKernelGGL(k_xpby<T>, grid1(n), dim3(256), 0, c.stream, n, v.z.p, cgs, cur, it + k == 0 ? 1 : 0,
-                           v.p.p);
-        spmv_t<T, T>(c, A, nullptr, v.p.p, v.q.p);
-        hipLaunchKernelGGL(k_dot<T>, dim3(nb), dim3(256), 0, c.stream, n, v.p.p, v.q.p, c.kw.part.p, 0);
-        hipLaunchKernelGGL(k_final, dim3(1), dim3(256), 0, c.stream, c.kw.part.p, nb, 1, 0, cgs + kCgPq);
-        hipLaunchKernelGGL(k_cg_update<T>, dim3(nb), dim3(256), 0, c.stream, n, cgs, cur, v.p.p, v.q.p, x, v.r.p,
-                           c.kw.part.p);
-        hipLaunchKernelGGL(k_final, dim3(1), dim3(256), 0, c.stream, c.kw.part.p, nb, 1, 0, cgs + kCgHist + k);
+        hipLaunchKernelGGL(k_final, dim3(1), dim3(256), 0, c.stream, v.part, nb, 1, 0, cgs + kCgRho + cur);
+        hipLaunchKernelGGL(k_xpby<T>, grid1(n), dim3(256), 0, c.stream, n, v.z, cgs, cur, it + k == 0 ? 1 : 0, v.p);
+        spmv_t<T, T>(c, A, nullptr, v.p, v.q);
+        hipLaunchKernelGGL(k_dot<T>, dim3(nb), dim3(256), 0, c.stream, n, v.p, v.q, v.part, 0);
+        hipLaunchKernelGGL(k_final, dim3(1), dim3(256), 0, c.stream, v.part, nb, 1, 0, cgs + kCgPq);
+        hipLaunchKernelGGL(k_cg_update<T>, dim3(nb), dim3(256), 0, c.stream, n, cgs, cur, v.p, v.q, x, v.r, v.part);
+        hipLaunchKernelGGL(k_final, dim3(1), dim3(256), 0, c.stream, v.part, nb, 1, 0, cgs + kCgHist + k);
       }
       MHX_HIP(hipGetLastError());
-      MHX_HIP(hipMemcpyAsync(hist.data(), cgs + kCgHist, w * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+      MHX_HIP(hipMemcpyAsync(hist, cgs + kCgHist, w * sizeof(double), hipMemcpyDeviceToHost, c.stream));
       MHX_HIP(hipStreamSynchronize(c.stream));
     }
     for (int k = 0; k < w; ++k) {
@@ -546,10 +561,21 @@ static int cg_core(Ctx& c, const T* A, const Prec<T>& P, const T* b, T* x, doubl
     }
   }
   // true residual
-  spmv_t<T, T>(c, A, nullptr, x, v.q.p);
-  hipLaunchKernelGGL(k_resid<T>, grid1(n), dim3(256), 0, c.stream, n, b, v.q.p, v.r.p);
-  *relres = norm2_t<T>(c, v.r.p, n) / bn;
+  spmv_t<T, T>(c, A, nullptr, x, v.q);
+  hipLaunchKernelGGL(k_resid<T>, grid1(n), dim3(256), 0, c.stream, n, b, v.q, v.r);
+  *relres = nrm(v.r) / bn;
   return it;
+}
+
+template <class T>
+static int cg_core(Ctx& c, const T* A, const Prec<T>& P, const T* b, T* x, double tol, int maxit, double* relres) {
+  const long n = c.mesh.ndof;
+  KrylovVecs<T>& v = vecs<T>(c);
+  v.r.alloc(n); v.z.alloc(n); v.p.alloc(n); v.q.alloc(n);
+  ensure_red(c, 1);
+  if (c.kw.cgs.n < (size_t)(kCgHist + 16)) c.kw.cgs.alloc(kCgHist + 16);
+  const CgBufs<T> bufs{v.r.p, v.z.p, v.p.p, v.q.p, c.kw.part.p, c.kw.cgs.p, c.kw.scal.p + kScalN};
+  return cg_run<T>(c, A, P, b, x, tol, maxit, relres, bufs, true);
 }
 
 // ---- GMRES(m), right preconditioning, CGS + DGKS re-orthogonalisation --------
@@ -777,6 +803,74 @@ static int mixed_solve(Ctx& c, const double* A, const double* b, double* x, cons
   *relres = rel;
   *refinements = refs;
   return total;
+}
+
+// ---- CG without device allocation -------------------------------------------
+// One file-scope device array per GPU holds the Krylov vectors (r, z, p, q),
+// the Jacobi scaling and the reduction partials/scalars of
+// krylov_solve_noalloc.  Its host-side name is only a symbol: the device
+// address is taken once per device with hipGetSymbolAddress and used
+// everywhere the host hands the memory to a kernel.
+constexpr long kNoAllocTail = kRedBlocks + 256;  // partials, CG scalars, one slot
+constexpr long kNoAllocDoubles = 5L * MHX_NOALLOC_MAX_DOF + kNoAllocTail;
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[kMaxDevices] = {};
+
+static double* scratch_ptr() {
+  int dev = 0;
+  MHX_HIP(hipGetDevice(&dev));
+  MHX_REQUIRE(dev >= 0 && dev < kMaxDevices, MHX_ERR_UNSUPPORTED, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    MHX_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    MHX_REQUIRE(p, MHX_ERR_DEVICE, "no device address for the solve scratch array");
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  return g_scratch_ptr[dev];
+}
+
+__global__ void k_fill(long n, double v, double* __restrict__ out) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = v;
+}
+
+int krylov_solve_noalloc(Ctx& c, const double* A, const double* b, double* x, const mhx_krylov_opts& o,
+                         mhx_solve_stats* st) {
+  const long n = c.mesh.ndof;
+  MHX_REQUIRE(n <= MHX_NOALLOC_MAX_DOF, MHX_ERR_ARG,
+              "allocation-free solve: n_dof exceeds MHX_NOALLOC_MAX_DOF (1048576)");
+  MHX_REQUIRE(o.solver == MHX_KRYLOV_CG, MHX_ERR_UNSUPPORTED, "allocation-free solve: CG only");
+  MHX_REQUIRE(o.precond == MHX_PREC_NONE || o.precond == MHX_PREC_JACOBI, MHX_ERR_UNSUPPORTED,
+              "allocation-free solve: no preconditioner or point Jacobi in FP64 only");
+  double* s = scratch_ptr();
+  const long N = MHX_NOALLOC_MAX_DOF;
+  CgBufs<double> v{s, s + N, s + 2 * N, s + 3 * N, s + 5 * N, s + 5 * N + kRedBlocks,
+                   s + 5 * N + kRedBlocks + 128};
+  double* minv = s + 4 * N;
+  // the array keeps the last call's contents: the CG scalars start cleared
+  hipLaunchKernelGGL(k_fill, dim3(1), dim3(256), 0, c.stream, 256L, 0.0, v.cgs);
+  MHX_HIP(hipGetLastError());
+  Prec<double> P;
+  P.kind = o.precond;
+  if (o.precond == MHX_PREC_JACOBI) {
+    hipLaunchKernelGGL(k_jacobi<double>, grid1(n), dim3(256), 0, c.stream, n, c.rowptr.p, c.colidx.p, A, minv);
+    MHX_HIP(hipGetLastError());
+    P.d = minv;
+  }
+  const int maxit = o.max_iters > 0 ? o.max_iters : 100;
+  const double tol = o.tol > 0 ? o.tol : 1e-7;
+  double rel = 0.0;
+  const int it = cg_run<double>(c, A, P, b, x, tol, maxit, &rel, v, false);
+  if (st) {
+    st->iters += it;
+    st->rel_res = rel;
+  }
+  if (!(rel <= tol)) return MHX_ERR_NOT_CONVERGED;
+  return MHX_OK;
 }
 
 int krylov_solve(Ctx& c, const double* A, const double* b, double* x, const mhx_krylov_opts& o, mhx_solve_stats* st) {

@@ -252,6 +252,9 @@ double norm2(Ctx& c, const double* a, long n);
 double norm_inf(Ctx& c, const double* a, long n);
 int krylov_solve(Ctx& c, const double* A, const double* b, double* x, const mhx_krylov_opts& o,
                  mhx_solve_stats* st);
+// CG / Jacobi-PCG with every device buffer in a static per-GPU array (no allocation)
+int krylov_solve_noalloc(Ctx& c, const double* A, const double* b, double* x, const mhx_krylov_opts& o,
+                         mhx_solve_stats* st);
 void set_dirichlet_dev(Ctx& c, double* u);
 void axpy(Ctx& c, long n, double a, const double* x, double* y);
 
