@@ -25,9 +25,11 @@
 namespace mhx {
 
 constexpr int kRedBlocks = 1024;  // partial sums per reduction
-constexpr int kScalH = 0;         // device scalars: Gram-Schmidt coefficients
-constexpr int kScalY = 96;        //                 GMRES least-squares solution
-constexpr int kScalN = 200;       //                 single reductions
+constexpr int kScalH = 0;         // device scalars: Gram-Schmidt coefficients, first pass
+constexpr int kScalH2 = 80;       //                 second (DGKS) pass
+constexpr int kScalW = 160;       //                 ||w||^2 after pass 1 and 2, DGKS flag, ||w||
+constexpr int kScalY = 168;       //                 GMRES least-squares solution
+constexpr int kScalN = 240;       //                 single reductions
 constexpr int kScalSize = 256;
 constexpr int kMaxRefine = 100;
 
@@ -143,25 +145,30 @@ __global__ __launch_bounds__(256) void k_dot(long n, const T* __restrict__ a, co
   }
 }
 
-// reduce K rows of nb partials (row stride kRedBlocks) into out[K]
-__global__ __launch_bounds__(256) void k_final(const double* __restrict__ part, int nb, int K, int mode,
-                                               double* __restrict__ out) {
-  __shared__ double sh[256];
-  for (int k = 0; k < K; ++k) {
-    double v = 0.0;
-    for (int i = threadIdx.x; i < nb; i += 256) {
-      double p = part[(long)k * kRedBlocks + i];
-      v = mode == 0 ? v + p : nanmax(v, p);
-    }
-    sh[threadIdx.x] = v;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-      if (threadIdx.x < s) sh[threadIdx.x] = mode == 0 ? sh[threadIdx.x] + sh[threadIdx.x + s] : nanmax(sh[threadIdx.x], sh[threadIdx.x + s]);
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) out[k] = sh[0];
-    __syncthreads();
+// reduce row k = blockIdx.x of nb partials (row stride kRedBlocks) into
+// out[k]; one block per row, so K values take one launch of K blocks.  flag
+// (optional): skip when *flag == 0 (a device-side decision, e.g. DGKS)
+__global__ __launch_bounds__(256) void k_final(const double* __restrict__ part, int nb, int mode,
+                                               double* __restrict__ out, const double* __restrict__ flag) {
+  if (flag && *flag == 0.0) return;
+  const int k = blockIdx.x;
+  double v = 0.0;
+  for (int i = threadIdx.x; i < nb; i += 256) {
+    const double p = part[(long)k * kRedBlocks + i];
+    v = mode == 0 ? v + p : nanmax(v, p);
   }
+  // fixed-shape reduction: within each wavefront, then over the four
+  if (mode == 0) {
+    v = wave_sum(v);
+  } else {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = nanmax(v, __shfl_xor(v, off, 64));
+  }
+  __shared__ double sh[4];
+  if (threadIdx.x % 64 == 0) sh[threadIdx.x / 64] = v;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    out[k] = mode == 0 ? (sh[0] + sh[1]) + (sh[2] + sh[3]) : nanmax(nanmax(sh[0], sh[1]), nanmax(sh[2], sh[3]));
 }
 
 static int red_blocks(long n) {
@@ -176,15 +183,13 @@ static void ensure_red(Ctx& c, int K) {
 
 // the nb partials at part reduced into *slot (device) and returned to the host
 static double fetch_to(Ctx& c, const double* part, double* slot, int nb, int mode) {
-  hipLaunchKernelGGL(k_final, dim3(1), dim3(256), 0, c.stream, part, nb, 1, mode, slot);
+  hipLaunchKernelGGL(k_final, dim3(1), dim3(256), 0, c.stream, part, nb, mode, slot, nullptr);
   MHX_HIP(hipGetLastError());
   double h;
   MHX_HIP(hipMemcpyAsync(&h, slot, sizeof(double), hipMemcpyDeviceToHost, c.stream));
   MHX_HIP(hipStreamSynchronize(c.stream));
   return h;
 }
-
-static double fetch(Ctx& c, int nb, int mode = 0) { return fetch_to(c, c.kw.part.p, c.kw.scal.p + kScalN, nb, mode); }
 
 template <class T>
 static double reduce_to(Ctx& c, long n, const T* a, const T* b, int mode, double* part, double* slot) {
@@ -541,13 +546,13 @@ static int cg_run(Ctx& c, const T* A, const Prec<T>& P, const T* b, T* x, double
           hipLaunchKernelGGL(k_prec_dot<T>, dim3(nb), dim3(256), 0, c.stream, n,
                              P.kind == MHX_PREC_JACOBI ? P.d : (const T*)nullptr, v.r, v.z, v.part);
         }
-        hipLaunchKernelGGL(k_final, dim3(1), dim3(256), 0, c.stream, v.part, nb, 1, 0, cgs + kCgRho + cur);
+        hipLaunchKernelGGL(k_final, dim3(1), dim3(256), 0, c.stream, v.part, nb, 0, cgs + kCgRho + cur, nullptr);
         hipLaunchKernelGGL(k_xpby<T>, grid1(n), dim3(256), 0, c.stream, n, v.z, cgs, cur, it + k == 0 ? 1 : 0, v.p);
         spmv_t<T, T>(c, A, nullptr, v.p, v.q);
         hipLaunchKernelGGL(k_dot<T>, dim3(nb), dim3(256), 0, c.stream, n, v.p, v.q, v.part, 0);
-        hipLaunchKernelGGL(k_final, dim3(1), dim3(256), 0, c.stream, v.part, nb, 1, 0, cgs + kCgPq);
+        hipLaunchKernelGGL(k_final, dim3(1), dim3(256), 0, c.stream, v.part, nb, 0, cgs + kCgPq, nullptr);
         hipLaunchKernelGGL(k_cg_update<T>, dim3(nb), dim3(256), 0, c.stream, n, cgs, cur, v.p, v.q, x, v.r, v.part);
-        hipLaunchKernelGGL(k_final, dim3(1), dim3(256), 0, c.stream, v.part, nb, 1, 0, cgs + kCgHist + k);
+        hipLaunchKernelGGL(k_final, dim3(1), dim3(256), 0, c.stream, v.part, nb, 0, cgs + kCgHist + k, nullptr);
       }
       MHX_HIP(hipGetLastError());
       MHX_HIP(hipMemcpyAsync(hist, cgs + kCgHist, w * sizeof(double), hipMemcpyDeviceToHost, c.stream));
@@ -582,7 +587,9 @@ static int cg_core(Ctx& c, const T* A, const Prec<T>& P, const T* b, T* x, doubl
 // h[k] = V_k . w for k < K  (V row-major: vector k at V + k*ld), FP64 sums
 template <class T>
 __global__ __launch_bounds__(256) void k_mdot(long n, int K, const T* __restrict__ V, long ld,
-                                              const T* __restrict__ w, double* __restrict__ part) {
+                                              const T* __restrict__ w, double* __restrict__ part,
+                                              const double* __restrict__ flag) {
+  if (flag && *flag == 0.0) return;
   // K <= 65 handled in chunks of 8 to bound registers
   __shared__ double sh[8][4];
   for (int k0 = 0; k0 < K; k0 += 8) {
@@ -610,7 +617,8 @@ __global__ __launch_bounds__(256) void k_mdot(long n, int K, const T* __restrict
 template <class T>
 __global__ __launch_bounds__(256) void k_mupdate(long n, int K, const T* __restrict__ V, long ld,
                                                  const double* __restrict__ h, T* __restrict__ w,
-                                                 double* __restrict__ part) {
+                                                 double* __restrict__ part, const double* __restrict__ flag) {
+  if (flag && *flag == 0.0) return;
   double v = 0.0;
   for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
     double wi = (double)w[i];
@@ -637,6 +645,26 @@ __global__ void k_vcomb(long n, int K, const T* __restrict__ V, long ld, const d
   out[i] = (T)s;
 }
 
+// DGKS on the device, after the first Gram-Schmidt pass: with h the K
+// coefficients and s[0] = ||w||^2 after the pass, ||w|| before it was
+// sqrt(s[0] + sum h^2); re-orthogonalise (s[2] = 1) when the norm dropped by
+// more than 1/sqrt(2).
+__global__ void k_dgks(int K, const double* __restrict__ h, double* __restrict__ s) {
+  if (threadIdx.x != 0) return;
+  double h2 = 0.0;
+  for (int j = 0; j < K; ++j) h2 += h[j] * h[j];
+  const double before = sqrt(s[0] + h2);
+  s[2] = sqrt(s[0]) > 0.7071067811865476 * before ? 0.0 : 1.0;
+}
+// ||w|| after the last pass (s[3]) and w /= ||w|| when it is nonzero
+template <class T>
+__global__ void k_normalize(long n, double* __restrict__ s, T* __restrict__ w) {
+  const double hn = sqrt(s[2] != 0.0 ? s[1] : s[0]);
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) s[3] = hn;
+  if (i < n && hn > 0.0) w[i] = (T)((1.0 / hn) * (double)w[i]);
+}
+
 template <class T>
 static int gmres_core(Ctx& c, const T* A, const Prec<T>& P, const T* b, T* x, int m, double tol, int maxit,
                       double* relres, bool stall = false) {
@@ -653,7 +681,7 @@ static int gmres_core(Ctx& c, const T* A, const Prec<T>& P, const T* b, T* x, in
     *relres = 0.0;
     return 0;
   }
-  std::vector<double> H((m + 1) * m), cs(m), sn(m), g(m + 1), hcol(m + 1), y(m), hh(m + 1);
+  std::vector<double> H((m + 1) * m), cs(m), sn(m), g(m + 1), hcol(m + 1), y(m), hh(kScalW + 4);
   double* dscal = c.kw.scal.p;
   int it = 0;
   double rel = 1.0;
@@ -679,29 +707,29 @@ static int gmres_core(Ctx& c, const T* A, const Prec<T>& P, const T* b, T* x, in
       T* vk = v.V.p + (long)k * n;
       T* vk1 = v.V.p + (long)(k + 1) * n;
       op_apply<T>(c, A, P, vk, vk1, v.tmp.p);  // w = A M^-1 v_k, stored in V_{k+1}
-      std::fill(hcol.begin(), hcol.end(), 0.0);
-      double wn0 = 0.0;
+      // classical Gram-Schmidt with one DGKS re-orthogonalisation pass, all
+      // decisions on the device; the host reads the coefficients of both
+      // passes, the flag and ||w|| with one copy per iteration
+      double* hs = dscal + kScalW;
       for (int pass = 0; pass < 2; ++pass) {
-        // h = V^T w on the device; w -= V h reads h there; one host sync per pass
-        hipLaunchKernelGGL(k_mdot<T>, dim3(nb), dim3(256), 0, c.stream, n, k + 1, v.V.p, n, vk1, c.kw.part.p);
-        hipLaunchKernelGGL(k_final, dim3(1), dim3(256), 0, c.stream, c.kw.part.p, nb, k + 1, 0, dscal + kScalH);
-        MHX_HIP(hipMemcpyAsync(hh.data(), dscal + kScalH, (k + 1) * sizeof(double), hipMemcpyDeviceToHost, c.stream));
-        hipLaunchKernelGGL(k_mupdate<T>, dim3(nb), dim3(256), 0, c.stream, n, k + 1, v.V.p, n, dscal + kScalH, vk1,
-                           c.kw.part.p);
-        const double wn2 = fetch(c, nb);
-        if (pass == 0) {
-          // ||w||^2 before the projection = ||w_after||^2 + sum h^2
-          for (int j = 0; j <= k; ++j) wn0 += hh[j] * hh[j];
-        }
-        for (int j = 0; j <= k; ++j) hcol[j] += hh[j];
-        hcol[k + 1] = std::sqrt(wn2);
-        // DGKS criterion: re-orthogonalise when the norm dropped by more than 1/sqrt(2)
-        const double before = std::sqrt(wn2 + (pass == 0 ? wn0 : 0.0));
-        if (pass == 0 && hcol[k + 1] > 0.7071067811865476 * before) break;
+        const double* fl = pass == 0 ? nullptr : hs + 2;
+        double* hp = dscal + (pass == 0 ? kScalH : kScalH2);
+        hipLaunchKernelGGL(k_mdot<T>, dim3(nb), dim3(256), 0, c.stream, n, k + 1, v.V.p, n, vk1, c.kw.part.p, fl);
+        hipLaunchKernelGGL(k_final, dim3(k + 1), dim3(256), 0, c.stream, c.kw.part.p, nb, 0, hp, fl);
+        hipLaunchKernelGGL(k_mupdate<T>, dim3(nb), dim3(256), 0, c.stream, n, k + 1, v.V.p, n, hp, vk1, c.kw.part.p,
+                           fl);
+        hipLaunchKernelGGL(k_final, dim3(1), dim3(256), 0, c.stream, c.kw.part.p, nb, 0, hs + pass, fl);
+        if (pass == 0) hipLaunchKernelGGL(k_dgks, dim3(1), dim3(64), 0, c.stream, k + 1, dscal + kScalH, hs);
       }
-      const double hn = hcol[k + 1];
+      hipLaunchKernelGGL(k_normalize<T>, grid1(n), dim3(256), 0, c.stream, n, hs, vk1);
+      MHX_HIP(hipGetLastError());
+      MHX_HIP(hipMemcpyAsync(hh.data(), dscal, (kScalW + 4) * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+      MHX_HIP(hipStreamSynchronize(c.stream));
+      const bool second = hh[kScalW + 2] != 0.0;
+      for (int j = 0; j <= k; ++j) hcol[j] = hh[kScalH + j] + (second ? hh[kScalH2 + j] : 0.0);
+      const double hn = hh[kScalW + 3];
+      hcol[k + 1] = hn;
       if (!std::isfinite(hn)) throw Error{MHX_ERR_NONFINITE, "GMRES: non-finite Krylov vector"};
-      if (hn > 0.0) hipLaunchKernelGGL(k_scale_to<T>, grid1(n), dim3(256), 0, c.stream, n, 1.0 / hn, vk1, vk1);
       for (int j = 0; j <= k + 1; ++j) H[j * m + k] = hcol[j];
       for (int j = 0; j < k; ++j) {
         const double a0 = H[j * m + k], b0 = H[(j + 1) * m + k];
