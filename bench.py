@@ -31,12 +31,17 @@ HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 LDS_PEAK_TBS = 256 * 128 * 2.4e9 / 1e12  # 256 CUs x 128 B/clk x 2.4 GHz
 
-# HBM traffic of k_thermal<jacres> (96^3 Q2, element colouring), per launch,
-# from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of this bench (mean of the
-# 8 colour launches), scaled by the factors that make the same counters read
-# the known size of the plain copy kernel: FETCH x 2048 B, WRITE x 1024 B.
-# Source: profiles/k_thermal_q2hex_96cubed.md.  Only valid for that workload.
-PMC_TRAFFIC = {(96, 0): {"fetch_kib": 529028.5, "write_kib": 1032647.0, "fetch_scale": 2048, "write_scale": 1024}}
+# HBM traffic per launch of the dominant assembly kernel (96^3 Q2), keyed by
+# (elements per axis, assembly.colouring), from rocprofv3 --pmc FETCH_SIZE /
+# WRITE_SIZE runs of this bench (mean of the 8 colour launches), scaled by the
+# factors that make the same counters read the known size of the plain copy
+# kernel: FETCH x 2048 B, WRITE x 1024 B.  Colouring 0: k_thermal<jacres>
+# (profiles/k_thermal_q2hex_96cubed.md); colouring 3: k_q2box<jacres>, x-segments
+# of 12 elements (profiles/k_q2box_q2hex_96cubed.md).  Only valid for those
+# workloads.
+PMC_TRAFFIC = {(96, 0): {"fetch_kib": 529028.5, "write_kib": 1032647.0, "fetch_scale": 2048, "write_scale": 1024},
+               (96, 3): {"fetch_kib": 327241.4, "write_kib": 722878.3, "fetch_scale": 2048, "write_scale": 1024,
+                         "segment": 12}}
 
 
 def c2_problem(n: int):
@@ -414,7 +419,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=96, help="elements per axis")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--colouring", type=int, default=0, help="0 element colouring, 1 block colouring")
+    ap.add_argument("--colouring", type=int, default=-1,
+                    help="assembly schedule: -1 the context's default (3, pencil streaming, on this box mesh), "
+                         "0 element colouring, 1 block colouring, 2 x-segments")
+    ap.add_argument("--segment", type=int, default=0, help="pencil streaming: elements per x-segment (0 = default)")
     ap.add_argument("--probe-gib", type=float, default=1.0, help="size of the plain-copy bandwidth probe")
     ap.add_argument("--skip-subgrid", action="store_true", help="skip the batched-subgrid measurement")
     ap.add_argument("--krylov-iters", type=int, default=100, help="GMRES iterations timed (0 = skip)")
@@ -435,7 +443,12 @@ def main():
     prob = c2_problem(n)
     t0 = time.perf_counter()
     g = Mhx(prob)
-    g.set_option("assembly.colouring", args.colouring)
+    if args.colouring >= 0:
+        g.set_option("assembly.colouring", args.colouring)
+    if args.segment > 0:
+        g.set_option("assembly.segment", args.segment)
+    colouring = g.get_option("assembly.colouring")
+    segment = g.get_option("assembly.segment")
     t_setup = time.perf_counter() - t0
     u = rng_fill(7, g.n_dof, 0, 0.0, 1.0)
     g.setDirichlet(u)
@@ -458,7 +471,7 @@ def main():
     ms_step = dt * 1e3 / args.steps
     mdofs = g.n_dof / (ms_step * 1e-3) / 1e6
 
-    kname = "k_thermal<jacres>"
+    kname = "k_q2box<jacres>" if colouring == 3 else "k_thermal<jacres>"
     k_ms, k_cnt = prof.get(kname, (float("nan"), 0))
     launches_per_step = k_cnt / args.steps
     mean_launch_ms = k_ms / max(k_cnt, 1)
@@ -467,7 +480,9 @@ def main():
     B_launch = B / launches_per_step
     achieved = B_launch / (mean_launch_ms * 1e-3) / 1e9
     B_min = 8 * g.nnz + 16 * g.n_dof + 8 * 2 * g.n_elem + 8 * 3 * n_vert
-    pmc = PMC_TRAFFIC.get((n, args.colouring))
+    pmc = PMC_TRAFFIC.get((n, colouring))
+    if pmc and pmc.get("segment", segment) != segment:
+        pmc = None  # measured at another segment length
     traffic = (pmc["fetch_kib"] * pmc["fetch_scale"] + pmc["write_kib"] * pmc["write_scale"]) if pmc else None
 
     # achievable-bandwidth reference: plain device copy of known size
@@ -517,10 +532,12 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "traffic_source": "rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE, copy-kernel calibrated" if pmc else None,
+                     "traffic_per": "launch",
                      "kernel": kname, "launches_per_step": launches_per_step,
                      "mean_launch_ms": round(mean_launch_ms, 5), "bytes_per_launch": B_launch,
                      "B_asm": B, "B_min": B_min, "copy_probe_GBps": probe,
-                     "colouring": "block" if args.colouring else "element"},
+                     "colouring": {0: "element", 1: "block", 2: "x-segments", 3: "pencil streaming"}[colouring],
+                     "segment": segment if colouring == 3 else None},
         "cpu_baseline": cpu,
         "phases_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in prof.items()},
         "krylov": krylov,

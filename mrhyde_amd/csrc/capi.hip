@@ -348,6 +348,10 @@ int mhx_ctx_create(const mhx_problem_desc* desc, mhx_ctx** out) {
   MHX_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   c->own_stream = true;
   build_box(*c);
+  // the lattice without node perturbation has axis-aligned box elements: Q2
+  // thermal assembly then defaults to the pencil-streamed kernel
+  c->box_mesh = desc->perturb == 0.0;
+  if (q2box_applicable(*c)) c->scatter_mode = kPencilStream;
   c->desc.k0 = nullptr;
   c->desc.src_elem = nullptr;
   c->desc.dbc_values = nullptr;
@@ -853,11 +857,20 @@ int mhx_set_option(mhx_ctx* ctx, const char* name, int64_t value) {
   if (!ctx || !name) return MHX_ERR_ARG;
   const std::string k(name);
   if (k == "assembly.colouring") {
-    if (value != kElementColouring && value != kBlockColouring && value != kSegmentMerge) {
-      ctx->err = "assembly.colouring must be 0, 1 or 2";
+    if (value != kElementColouring && value != kBlockColouring && value != kSegmentMerge &&
+        !(value == kPencilStream && q2box_applicable(*ctx))) {
+      ctx->err = "assembly.colouring must be 0, 1 or 2 (3: Q2 thermal on a box mesh only)";
       return MHX_ERR_ARG;
     }
     ctx->scatter_mode = (int)value;
+    return MHX_OK;
+  }
+  if (k == "assembly.segment") {
+    if (value < 2 || value > 4096 || (value & 1)) {
+      ctx->err = "assembly.segment must be an even element count in [2, 4096]";
+      return MHX_ERR_ARG;
+    }
+    ctx->q2_segment = (int)value;
     return MHX_OK;
   }
   if (k == "krylov.spmv_lanes") {
@@ -915,6 +928,10 @@ int mhx_set_option(mhx_ctx* ctx, const char* name, int64_t value) {
 int mhx_get_option(const mhx_ctx* ctx, const char* name, int64_t* value) {
   if (!ctx || !name || !value) return MHX_ERR_ARG;
   const std::string k(name);
+  if (k == "assembly.segment") {
+    *value = ctx->q2_segment;
+    return MHX_OK;
+  }
   if (k == "assembly.colouring") {
     *value = ctx->scatter_mode;
     return MHX_OK;

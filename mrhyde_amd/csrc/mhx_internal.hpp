@@ -95,7 +95,7 @@ struct AsmArgs {
   double rho_cp;          // density * specific heat
 };
 
-enum ScatterMode { kElementColouring = 0, kBlockColouring = 1, kSegmentMerge = 2 };
+enum ScatterMode { kElementColouring = 0, kBlockColouring = 1, kSegmentMerge = 2, kPencilStream = 3 };
 
 // ---------------------------------------------------------------------------
 // device buffers
@@ -218,6 +218,8 @@ struct Ctx {
   int amg_omega_pct = 40;  // block-Jacobi smoother: damping in percent
   int nfield = 1;  // unknowns per node (consecutive DOFs): block size of block Jacobi
   int scatter_mode = kElementColouring;
+  bool box_mesh = false;  // every element an axis-aligned box (structured box, no node perturbation)
+  int q2_segment = 12;    // pencil streaming: elements per x-segment (even)
   std::shared_ptr<Amg> amg;  // built by a solve with the AMG preconditioner, kept for reuse of its buffers
   // time-integration state (mhx_set_time_state); bdf_order 0 = steady
   int bdf_order = 0;
@@ -233,6 +235,9 @@ void lagrange_equispaced(int p, double xi, double* L, double* dL);
 
 // assembly (assemble.hip)
 bool thermal_supported(int dim, int p, int nq1);
+// Q2 hexahedra on box meshes: pencil-streamed pair-factorised assembly (assemble_q2box.hip)
+bool q2box_applicable(const Ctx& c);
+void launch_q2box(Ctx& c, AsmArgs& a, bool jac);
 void assemble_thermal(Ctx& c, const double* u, double* res, double* jac, bool want_res, bool want_jac,
                       bool accumulate);
 // thermal Neumann boundary residual (bc.hip)

@@ -405,3 +405,188 @@ def test_segment_loops_visit_every_slot_once(dim, p, block):
     if block == 128:
         bad = strided_visits(sc["kSize"], 128, 256)
         assert 0 in bad
+
+
+# ---------------------------------------------------------------------------
+# pencil streaming (assemble_q2box.hip, assembly.colouring = 3, Q2 hexahedra):
+# a 64-lane wavefront walks an x-segment of one element pencil two elements
+# per step.  Restated from the kernel's round / lane arithmetic: every write
+# is (entry, contributing elements, first-touch flag); the replay checks that
+# each CRS entry receives the contribution of every element holding both of
+# its nodes exactly once, has exactly one first writer that runs before all
+# other writers, and that no two writers of an entry can run at once.
+# ---------------------------------------------------------------------------
+def pencil_units(nel, seg):
+    nseg = (nel[0] + seg - 1) // seg
+    for col in range(8):
+        cx, cy, cz = col & 1, (col >> 1) & 1, (col >> 2) & 1
+        cn = [(nseg - cx + 1) // 2, (nel[1] - cy + 1) // 2, (nel[2] - cz + 1) // 2]
+        for t in range(cn[0] * cn[1] * cn[2]):
+            tt = t
+            sx = 2 * (tt % cn[0]) + cx
+            tt //= cn[0]
+            ey = 2 * (tt % cn[1]) + cy
+            tt //= cn[1]
+            ez = 2 * tt + cz
+            yield col, t, sx, ey, ez, nseg
+
+
+def pencil_writes(nel, seg):
+    """[(launch, wg, entry, contributors, first)] for the Jacobian and
+    [(launch, wg, row, contributors, first)] for the residual."""
+    nn = [2 * n + 1 for n in nel]
+    jw, rw = [], []
+    for col, wg, sx, ey, ez, nseg in pencil_units(nel, seg):
+        x0 = sx * seg
+        x1 = min(x0 + seg, nel[0])
+        flx, fhx = sx == 0 or sx % 2 == 0, sx == nseg - 1 or sx % 2 == 0
+        fly, fhy = ey == 0 or ey % 2 == 0, ey == nel[1] - 1 or ey % 2 == 0
+        flz, fhz = ez == 0 or ez % 2 == 0, ez == nel[2] - 1 or ez % 2 == 0
+        Y0, Z0 = 2 * ey, 2 * ez
+        carry = {}   # (r, l) -> contributions of B's last-plane run, by column
+        rcarry = {}  # l -> residual contributions
+        for xa in range(x0, x1, 2):
+            hasB = xa + 1 < x1
+            first_step, last_step = xa == x0, xa + 2 >= x1
+            el = {0: (xa, ey, ez), 1: (xa + 1, ey, ez)}
+            # residual (lane l < 27 of half h: test function i = l)
+            R = {}
+            for h in (0, 1):
+                if h == 1 and not hasB:
+                    continue
+                for l in range(27):
+                    R[(h, l)] = [el[h]]
+            newr = {}
+            for h in (0, 1):
+                if h == 1 and not hasB:
+                    continue
+                ex = xa + h
+                for l in range(27):
+                    i1, i2, i3 = l % 3, (l // 3) % 3, l // 9
+                    X = 2 * ex + i1
+                    contrib = list(R[(h, l)])
+                    write, xsh, xfirst = True, False, True
+                    if h == 0 and i1 == 2:
+                        if hasB:
+                            contrib = contrib + R[(1, l - 2)]
+                        elif x1 < nel[0]:
+                            xsh, xfirst = True, fhx
+                    elif h == 0 and i1 == 0:
+                        if not first_step:
+                            contrib = rcarry[l + 2] + contrib
+                        elif x0 > 0:
+                            xsh, xfirst = True, flx
+                    elif h == 1 and i1 == 0:
+                        write = False
+                    elif h == 1 and i1 == 2:
+                        newr[l] = contrib
+                        write = last_step
+                        if last_step and x1 < nel[0]:
+                            xsh, xfirst = True, fhx
+                    if write:
+                        ysh, zsh = i2 != 1, i3 != 1
+                        first = (not xsh or xfirst) and (not ysh or (fly if i2 == 0 else fhy)) and \
+                                (not zsh or (flz if i3 == 0 else fhz))
+                        rw.append((col, wg, (X, Y0 + i2, Z0 + i3), tuple(contrib), first))
+            rcarry.update(newr)
+            # Jacobian rounds
+            for r in range(9):
+                m = r // 3
+                for h in (0, 1):
+                    if h == 1 and not hasB:
+                        continue
+                    ex = xa + h
+                    for l in range(32):
+                        pp = l + 32 * (r % 3)
+                        if pp >= 81:
+                            continue
+                        i1 = m if h else 2 - m
+                        j2, j3, i2, i3 = pp % 3, (pp // 3) % 3, (pp // 9) % 3, pp // 27
+                        X = 2 * ex + i1
+                        # this lane's run: columns 2 ex + j1, j1 = 0..2, element ex
+                        own = {2 * ex + j1: [el[h]] for j1 in range(3)}
+                        cols = {}
+                        xsh_first, xsh_last = False, False
+                        if m == 1:
+                            cols = own
+                        elif m == 0:
+                            if h == 0:
+                                cols = dict(own)
+                                if hasB:
+                                    cols[2 * ex + 2] = own[2 * ex + 2] + [el[1]]
+                                else:
+                                    xsh_last = x1 < nel[0]
+                            else:
+                                cols = {2 * ex + 1: own[2 * ex + 1], 2 * ex + 2: own[2 * ex + 2]}
+                        elif h == 0:
+                            if not first_step:
+                                c = carry[(r, l)]
+                                cols = {2 * ex - 2: c[2 * ex - 2], 2 * ex - 1: c[2 * ex - 1],
+                                        2 * ex: c[2 * ex] + own[2 * ex], 2 * ex + 1: own[2 * ex + 1],
+                                        2 * ex + 2: own[2 * ex + 2]}
+                            else:
+                                cols = own
+                                xsh_first = x0 > 0
+                        else:
+                            carry[(r, l)] = own
+                            if last_step:
+                                cols = own
+                                xsh_last = x1 < nel[0]
+                        if not cols:
+                            continue
+                        ysh = i2 == j2 and i2 != 1
+                        zsh = i3 == j3 and i3 != 1
+                        yz = (not ysh or (fly if i2 == 0 else fhy)) and (not zsh or (flz if i3 == 0 else fhz))
+                        ks = sorted(cols)
+                        for k, cxv in enumerate(ks):
+                            first = yz
+                            if xsh_first and k == 0:
+                                first = first and flx
+                            if xsh_last and k == len(ks) - 1:
+                                first = first and fhx
+                            ent = ((X, Y0 + i2, Z0 + i3), (cxv, Y0 + j2, Z0 + j3))
+                            jw.append((col, wg, ent, tuple(cols[cxv]), first))
+    return jw, rw
+
+
+def q2_node_elems(node, nel):
+    """Elements holding a Q2 lattice node."""
+    out = []
+    rng = []
+    for d in range(3):
+        c = node[d]
+        cand = [c // 2 - 1, c // 2] if c % 2 == 0 else [c // 2]
+        rng.append([e for e in cand if 0 <= e < nel[d]])
+    for e in itertools.product(*rng):
+        out.append(e)
+    return out
+
+
+@pytest.mark.parametrize("seg", [2, 4, 6, 12])
+@pytest.mark.parametrize("nel", [(5, 4, 3), (7, 3, 2), (8, 1, 1), (1, 1, 1), (2, 5, 7), (13, 2, 3)])
+def test_pencil_schedule(nel, seg):
+    jw, rw = pencil_writes(nel, seg)
+    for writes, is_res in ((jw, False), (rw, True)):
+        by = {}
+        for col, wg, ent, contrib, first in writes:
+            by.setdefault(ent, []).append((col, wg, contrib, first))
+        if is_res:
+            expect = {tuple(n) for n in itertools.product(*(range(2 * n + 1) for n in nel))}
+        else:
+            expect = pattern_entries(3, 2, nel)
+        assert set(by) == expect, "writes do not cover the pattern exactly"
+        for ent, ws in by.items():
+            # each element holding both nodes contributes exactly once
+            if is_res:
+                want = sorted(q2_node_elems(ent, nel))
+            else:
+                er, ec_ = set(q2_node_elems(ent[0], nel)), set(q2_node_elems(ent[1], nel))
+                want = sorted(er & ec_)
+            got = sorted(e for w in ws for e in w[2])
+            assert got == want, f"{ent}: contributions {got} != {want}"
+            # concurrency: one writer per entry per launch (across and within workgroups)
+            launches = [w[0] for w in ws]
+            assert len(launches) == len(set(launches)), f"{ent}: two writers in one launch {ws}"
+            firsts = [w for w in ws if w[3]]
+            assert len(firsts) == 1, f"{ent}: {len(firsts)} first writers {ws}"
+            assert all(w[0] > firsts[0][0] for w in ws if not w[3]), f"{ent}: a later touch precedes the first"
