@@ -133,6 +133,26 @@ typedef struct mhx_problem_desc {
   double  ns_force[3];    /* "source ux", "source uy", "source uz" (constants) */
   int32_t ns_pin_pressure;/* 1: pr = 0 at the corner node (xmin, ymin[, zmin]) */
   uint32_t ns_comp_sides[3];
+
+  /* Coefficient functions: the deck's "Functions" block, each entry
+   * registered by name as FunctionManager::addFunction(name, expression,
+   * "ip") does (src/managers/function/functionManager_create.hpp:14-39,
+   * src/interfaces/physics/physicsInterface_functions.hpp:94-150).  The
+   * thermal module reads "thermal diffusion", "thermal source",
+   * "specific heat", "density", "advection x|y|z" (src/physics/
+   * thermal.cpp:55-61; defaults 1, 0, 1, 1, 0); a given one replaces the
+   * closed form above (k0, diff_u2, src_elem, modes, th_density,
+   * th_specific_heat).  Other names are functions the expressions may use.
+   * Expressions follow the reference interpreter (operators + - * / ^ < <=
+   * > >=, sin cos exp log tan abs max min mean sqrt sinh cosh, numbers, pi)
+   * over "T", "grad(T)[x|y|z]", "T_t", "x", "y", "z", "t" (fn_time) and the
+   * element data "k0" and "src" where given.  Compiled at mhx_ctx_create;
+   * MHX_ERR_ARG with a message for what cannot be compiled. */
+  int32_t n_functions;
+  int32_t fn_reserved;
+  const char* const* function_names;   /* [n_functions] */
+  const char* const* function_exprs;   /* [n_functions] */
+  double  fn_time;                     /* value of "t" (set the stage time with mhx_ctx_update) */
 } mhx_problem_desc;
 
 typedef struct mhx_ctx mhx_ctx;
@@ -369,6 +389,16 @@ int  mhx_l2_error(mhx_ctx* ctx, const double* u, int32_t n_modes, const double* 
  *                         (default 40).
  * Returns MHX_ERR_ARG for an unknown name or value. */
 int  mhx_set_option(mhx_ctx* ctx, const char* name, int64_t value);
+
+/* FunctionManager::evaluate(name, "point") on the host
+ * (src/managers/function/functionManager_evaluate.hpp:14-50): compiles the
+ * descriptor's function `name` as mhx_ctx_create does and evaluates it at n
+ * points.  pts[11*i ..]: T, T_t, grad(T)[x], [y], [z], x, y, z, t, k0, src.
+ * out[6*i ..]: value, then its derivatives by T, grad(T)[x], [y], [z] and
+ * T_t.  Needs no GPU.  MHX_ERR_ARG with mhx_expr_last_error() for an
+ * expression that does not compile. */
+int  mhx_expr_eval(const mhx_problem_desc* desc, const char* name, int64_t n, const double* pts, double* out);
+const char* mhx_expr_last_error(void);
 int  mhx_get_option(const mhx_ctx* ctx, const char* name, int64_t* value);
 
 /* ---- device-resident state --------------------------------------------

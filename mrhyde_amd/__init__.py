@@ -6,7 +6,7 @@ without the GPU library.  Everything else lives in libmhx.so; accessing
 """
 from .abi import Problem, rng_fill  # noqa: F401
 
-_LAZY = {"Mhx", "MhxError", "krylov_opts", "rng", "SubgridBatch", "Server"}
+_LAZY = {"Mhx", "MhxError", "krylov_opts", "rng", "SubgridBatch", "Server", "function_eval"}
 
 
 def __getattr__(name):

@@ -62,6 +62,9 @@ class ProblemDesc(C.Structure):
         ("ns_lid_sides", C.c_uint32), ("ns_lid_u", C.c_double * 3),
         ("ns_force", C.c_double * 3), ("ns_pin_pressure", C.c_int32),
         ("ns_comp_sides", C.c_uint32 * 3),
+        ("n_functions", C.c_int32), ("fn_reserved", C.c_int32),
+        ("function_names", C.POINTER(C.c_char_p)), ("function_exprs", C.POINTER(C.c_char_p)),
+        ("fn_time", C.c_double),
     ]
 
 
@@ -201,6 +204,11 @@ class Problem:
     force: Sequence[float] = (0.0, 0.0, 0.0)
     pin_pressure: bool = False
     comp_sides: Sequence[int] = (0, 0, 0)  # extra Dirichlet sides per velocity component
+    # coefficient functions, the deck's "Functions" block: name -> expression
+    # ("thermal diffusion", "thermal source", "specific heat", "density",
+    # "advection x|y|z", or helper names they use); time = value of "t"
+    functions: dict = field(default_factory=dict)
+    time: float = 0.0
     _keep: list = field(default_factory=list, repr=False)
 
     @property
@@ -282,5 +290,14 @@ class Problem:
             d.ns_force[a] = float((list(self.force) + [0.0] * 3)[a])
             d.ns_comp_sides[a] = int((list(self.comp_sides) + [0] * 3)[a])
         d.ns_pin_pressure = int(bool(self.pin_pressure))
+        names = [str(k).encode() for k in self.functions]
+        exprs = [str(v).encode() for v in self.functions.values()]
+        d.n_functions = len(names)
+        if names:
+            na, ea = (C.c_char_p * len(names))(*names), (C.c_char_p * len(exprs))(*exprs)
+            self._keep.extend([names, exprs, na, ea])
+            d.function_names = C.cast(na, C.POINTER(C.c_char_p))
+            d.function_exprs = C.cast(ea, C.POINTER(C.c_char_p))
+        d.fn_time = float(self.time)
         return d
 

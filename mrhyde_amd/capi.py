@@ -69,6 +69,8 @@ def _load() -> C.CDLL:
         "mhx_read_matrix_market": (C.c_int, [vp, C.c_char_p, vp, i32]),
         "mhx_read_vector_market": (C.c_int, [vp, C.c_char_p, vp, i32]),
         "mhx_amg_level": (C.c_int, [vp, i32, i32, C.POINTER(i64), C.POINTER(C.c_double), vp, vp, vp]),
+        "mhx_expr_eval": (C.c_int, [C.POINTER(ProblemDesc), C.c_char_p, i64, vp, vp]),
+        "mhx_expr_last_error": (C.c_char_p, []),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -89,5 +91,5 @@ EXPORTED = [
     "mhx_footprint", "mhx_ctx_update", "mhx_server_create", "mhx_server_submit", "mhx_server_poll",
     "mhx_server_info", "mhx_server_last_error", "mhx_server_destroy", "mhx_amg_setup", "mhx_amg_level",
     "mhx_set_time_state", "mhx_write_matrix_market", "mhx_write_vector_market", "mhx_read_matrix_market",
-    "mhx_read_vector_market", "mhx_linear_solve_noalloc",
+    "mhx_read_vector_market", "mhx_linear_solve_noalloc", "mhx_expr_eval", "mhx_expr_last_error",
 ]

@@ -643,7 +643,7 @@ __global__ __launch_bounds__(64) void k_q2box(AsmArgs args) {
 // ---------------------------------------------------------------------------
 bool q2box_applicable(const Ctx& c) {
   return c.mesh.dim == 3 && c.mesh.p == 2 && c.mesh.nq1 == 3 && c.mesh.nf == 1 && c.box_mesh &&
-         c.desc.physics == MHX_PHYS_THERMAL;
+         c.desc.physics == MHX_PHYS_THERMAL && !c.xh.active;
 }
 
 void launch_q2box(Ctx& c, AsmArgs& a, bool jac) {

@@ -60,8 +60,26 @@ Profiler::~Profiler() {
 }
 
 // ---- helpers -----------------------------------------------------------------
+// compiled coefficient functions to the device (code and constants)
+static void upload_functions(Ctx& c) {
+  if (!c.xh.active) {
+    c.x_code.release();
+    c.x_consts.release();
+    return;
+  }
+  c.x_code.alloc(c.xh.code.size());
+  MHX_HIP(hipMemcpyAsync(c.x_code.p, c.xh.code.data(), c.xh.code.size() * sizeof(ExprIns), hipMemcpyHostToDevice,
+                         c.stream));
+  c.x_consts.alloc(c.xh.consts.empty() ? 1 : c.xh.consts.size());
+  if (!c.xh.consts.empty())
+    MHX_HIP(hipMemcpyAsync(c.x_consts.p, c.xh.consts.data(), c.xh.consts.size() * sizeof(double),
+                           hipMemcpyHostToDevice, c.stream));
+  MHX_HIP(hipStreamSynchronize(c.stream));
+}
+
 static void validate(const mhx_problem_desc* d) {
   MHX_REQUIRE(d, MHX_ERR_ARG, "null problem description");
+  MHX_REQUIRE(d->n_functions >= 0 && d->n_functions <= 256, MHX_ERR_ARG, "n_functions out of range");
   MHX_REQUIRE(d->dim == 2 || d->dim == 3, MHX_ERR_ARG, "dim must be 2 or 3");
   MHX_REQUIRE(d->order >= 1 && d->order <= kMaxP1 - 1, MHX_ERR_ARG, "order out of range");
   MHX_REQUIRE(d->quadrature >= 0 && d->quadrature / 2 + 1 <= kMaxQ1, MHX_ERR_ARG, "quadrature out of range");
@@ -344,10 +362,17 @@ int mhx_ctx_create(const mhx_problem_desc* desc, mhx_ctx** out) {
   MHX_REQUIRE(ndev > 0, MHX_ERR_DEVICE, "no HIP device");
   c = new mhx_ctx();
   c->desc = *desc;
+  // coefficient functions: compiled before any device work
+  expr_compile(*desc, c->xh);
+  MHX_REQUIRE(!c->xh.active || desc->physics == MHX_PHYS_THERMAL, MHX_ERR_UNSUPPORTED,
+              "coefficient functions are built for the thermal module");
+  MHX_REQUIRE(!c->xh.active || !desc->wdbc_sides, MHX_ERR_UNSUPPORTED,
+              "weak Dirichlet sides take the closed-form diffusion; not built with coefficient functions");
   // arrays in the description are consumed at create time only
   MHX_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   c->own_stream = true;
   build_box(*c);
+  upload_functions(*c);
   // the lattice without node perturbation has axis-aligned box elements: Q2
   // thermal assembly then defaults to the pencil-streamed kernel
   c->box_mesh = desc->perturb == 0.0;
@@ -355,6 +380,8 @@ int mhx_ctx_create(const mhx_problem_desc* desc, mhx_ctx** out) {
   c->desc.k0 = nullptr;
   c->desc.src_elem = nullptr;
   c->desc.dbc_values = nullptr;
+  c->desc.function_names = nullptr;
+  c->desc.function_exprs = nullptr;
   *out = c;
   return MHX_OK;
   }
@@ -825,6 +852,16 @@ int mhx_ctx_update(mhx_ctx* ctx, const mhx_problem_desc* d) {
   };
   upload(d->k0, c.k0);
   upload(d->src_elem, c.src);
+  // coefficient functions (new expressions or a new time)
+  ExprHost xh;
+  expr_compile(*d, xh);
+  MHX_REQUIRE(!xh.active || d->physics == MHX_PHYS_THERMAL, MHX_ERR_UNSUPPORTED,
+              "coefficient functions are built for the thermal module");
+  MHX_REQUIRE(!xh.active || !d->wdbc_sides, MHX_ERR_UNSUPPORTED,
+              "weak Dirichlet sides take the closed-form diffusion; not built with coefficient functions");
+  c.xh = std::move(xh);
+  upload_functions(c);
+  if (c.xh.active && c.scatter_mode == kPencilStream) c.scatter_mode = kElementColouring;
   // Dirichlet values on the (unchanged) Dirichlet DOFs
   const bool ns = d->physics == MHX_PHYS_NAVIER_STOKES;
   for (int32_t g : c.h_dbc_list) {
@@ -848,6 +885,8 @@ int mhx_ctx_update(mhx_ctx* ctx, const mhx_problem_desc* d) {
   c.desc.k0 = nullptr;
   c.desc.src_elem = nullptr;
   c.desc.dbc_values = nullptr;
+  c.desc.function_names = nullptr;
+  c.desc.function_exprs = nullptr;
   c.jac_valid = false;
   return MHX_OK;
   MHX_GUARD_END(ctx)

@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "../../include/mhx.h"
+#include "expr.hpp"
 
 namespace mhx {
 
@@ -93,7 +94,19 @@ struct AsmArgs {
   const double* hist;     // [ndof] nodal sum_k BDF_k u_prev[k-1] / dt, or nullptr (steady)
   double alpha_t;         // BDF_0 / dt
   double rho_cp;          // density * specific heat
+  ExprSet xp;             // compiled coefficient functions (xp.active = 0: the closed forms)
 };
+
+// compiled coefficient functions of a context, host side (expr.hip)
+struct ExprHost {
+  std::vector<ExprIns> code;
+  std::vector<double> consts;
+  int off[XF_COUNT] = {};
+  int len[XF_COUNT] = {};
+  double dflt[XF_COUNT] = {};
+  int active = 0;
+};
+void expr_compile(const mhx_problem_desc& d, ExprHost& out);
 
 enum ScatterMode { kElementColouring = 0, kBlockColouring = 1, kSegmentMerge = 2, kPencilStream = 3 };
 
@@ -225,6 +238,10 @@ struct Ctx {
   int bdf_order = 0;
   double alpha_t = 0.0;
   DevBuf<double> hist;       // [ndof] nodal history term of dT/dt
+  // compiled coefficient functions
+  ExprHost xh;
+  DevBuf<ExprIns> x_code;
+  DevBuf<double> x_consts;
   std::string err;
 };
 
@@ -238,6 +255,9 @@ bool thermal_supported(int dim, int p, int nq1);
 // Q2 hexahedra on box meshes: pencil-streamed pair-factorised assembly (assemble_q2box.hip)
 bool q2box_applicable(const Ctx& c);
 void launch_q2box(Ctx& c, AsmArgs& a, bool jac);
+// thermal assembly with compiled coefficient functions, element colouring
+// (assemble_expr.hip)
+void launch_thermal_expr(Ctx& c, AsmArgs& a, bool jac);
 void assemble_thermal(Ctx& c, const double* u, double* res, double* jac, bool want_res, bool want_jac,
                       bool accumulate);
 // thermal Neumann boundary residual (bc.hip)

@@ -60,6 +60,8 @@ struct Request {
   int64_t id = 0;
   mhx_problem_desc desc{};
   std::vector<double> k0, src, dbc;  // owned copies of the description's arrays
+  std::vector<std::string> fn_name, fn_expr;  // and of its coefficient functions
+  std::vector<const char*> fn_name_p, fn_expr_p;
   Key key;
   int64_t n_dof = 0, bytes = 0;
   clk::time_point t_submit, t_start, t_end;
@@ -238,6 +240,20 @@ int mhx_server_submit(mhx_server* s, const mhx_problem_desc* desc, int64_t* tick
   if (desc->dbc_values) {
     r->dbc.assign(desc->dbc_values, desc->dbc_values + r->n_dof);
     r->desc.dbc_values = r->dbc.data();
+  }
+  if (desc->n_functions > 0) {
+    if (!desc->function_names || !desc->function_exprs) return MHX_ERR_ARG;
+    for (int i = 0; i < desc->n_functions; ++i) {
+      if (!desc->function_names[i] || !desc->function_exprs[i]) return MHX_ERR_ARG;
+      r->fn_name.emplace_back(desc->function_names[i]);
+      r->fn_expr.emplace_back(desc->function_exprs[i]);
+    }
+    for (int i = 0; i < desc->n_functions; ++i) {
+      r->fn_name_p.push_back(r->fn_name[i].c_str());
+      r->fn_expr_p.push_back(r->fn_expr[i].c_str());
+    }
+    r->desc.function_names = r->fn_name_p.data();
+    r->desc.function_exprs = r->fn_expr_p.data();
   }
   r->key = key_of(*desc);
   r->t_submit = clk::now();

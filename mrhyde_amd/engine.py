@@ -56,6 +56,21 @@ def _ptr(a):
     raise TypeError(f"unsupported array type {type(a)}")
 
 
+def function_eval(problem: Problem, name: str, pts) -> np.ndarray:
+    """FunctionManager::evaluate(name, "point") on the host
+    (functionManager_evaluate.hpp:14-50): function `name` of
+    problem.functions, compiled as the assembly compiles it, at the rows of
+    pts [n, 11] (T, T_t, grad(T)[x], [y], [z], x, y, z, t, k0, src).
+    Returns [n, 6]: value and d/dT, d/dgrad(T)[x|y|z], d/dT_t."""
+    pts = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 11)
+    out = np.empty((pts.shape[0], 6))
+    d = problem.desc()
+    rc = lib.mhx_expr_eval(C.byref(d), name.encode(), pts.shape[0], pts.ctypes.data, out.ctypes.data)
+    if rc != 0:
+        raise MhxError(rc, lib.mhx_expr_last_error().decode())
+    return out
+
+
 def krylov_opts(solver: str = "gmres", tol: float = 1e-10, max_iters: int = 1000, restart: int = 50,
                 precond: str = "jacobi", device: bool = False, inner_iters: int = 0,
                 inner_tol: float = 0.0) -> capi.KrylovOpts:

@@ -46,6 +46,7 @@ def _load():
         "orc_dof_coords": (C.c_int, [vp, vp]),
         "orc_set_time_state": (C.c_int, [vp, C.c_double, i32, vp]),
         "orc_update": (C.c_int, [vp, C.POINTER(ProblemDesc)]),
+        "orc_expr_eval": (C.c_int, [C.POINTER(ProblemDesc), C.c_char_p, i64, vp, vp, i32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -61,6 +62,21 @@ def gauss(n: int):
     w = np.empty(n)
     lib.orc_gauss(n, x.ctypes.data, w.ctypes.data)
     return x, w
+
+
+def expr_eval(problem: Problem, name: str, pts: np.ndarray, one_element: bool = False) -> np.ndarray:
+    """The oracle's FunctionManager::evaluate of function `name` (defined in
+    problem.functions) at the rows of pts [n, 11] (T, T_t, grad(T)[x], [y],
+    [z], x, y, z, t, k0, src); each row its own element, or all rows one
+    element.  Returns [n, 6]: value and d/dT, d/dgrad(T)[x|y|z], d/dT_t."""
+    pts = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 11)
+    out = np.empty((pts.shape[0], 6))
+    d = problem.desc()
+    rc = lib.orc_expr_eval(C.byref(d), name.encode(), pts.shape[0], pts.ctypes.data, out.ctypes.data,
+                           int(one_element))
+    if rc != 0:
+        raise RuntimeError(lib.orc_last_error().decode())
+    return out
 
 
 class Oracle:

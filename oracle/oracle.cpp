@@ -75,6 +75,12 @@ template <class T> struct Seed;
 template <int N> struct Seed<SFad<N>> { static SFad<N> make(int idx, double x) { return SFad<N>(N, idx, x); } };
 template <> struct Seed<double> { static double make(int, double x) { return x; } };
 
+}  // namespace orc
+
+#include "functions.hpp"
+
+namespace orc {
+
 // ---------------------------------------------------------------------------
 // 1-D rules
 // ---------------------------------------------------------------------------
@@ -157,6 +163,8 @@ struct Problem {
   int nel[3], nn[3], nvx[3];         // elements, DOF nodes, vertices per axis
   long ne, ndof, nvert, nnode;       // ndof = nnode * nf, DOF = node * nf + field
   std::vector<double> k0, src;       // per element
+  FManager fm;                       // coefficient functions (FunctionManager)
+  bool fm_active = false;            // a thermal coefficient is given as a function
   std::vector<double> vx;            // vertex coords [nvert*3]
   std::vector<int> lids;             // [ne*nf*nb], (e, field, basis)
   std::vector<double> qx1, qw1;      // 1-D Gauss points/weights
@@ -182,6 +190,23 @@ struct Problem {
 };
 
 static std::string g_err;
+
+// the deck's coefficient functions (physicsInterface_functions.hpp:94-150,
+// thermal.cpp:55-61): every entry registered, decomposed at once
+static const char* const kThermalFunctions[] = {"thermal diffusion", "thermal source", "specific heat", "density",
+                                                "advection x", "advection y", "advection z"};
+static void set_functions(Problem& P, const mhx_problem_desc& d) {
+  P.fm = FManager();
+  P.fm_active = false;
+  if (d.n_functions <= 0) return;
+  if (!d.function_names || !d.function_exprs) ffail("function names/expressions missing");
+  P.fm.have_k0 = d.k0 != nullptr;
+  P.fm.have_src = d.src_elem != nullptr;
+  for (int i = 0; i < d.n_functions; ++i) P.fm.add(d.function_names[i], d.function_exprs[i]);
+  P.fm.decompose();
+  for (const char* nm : kThermalFunctions)
+    if (P.fm.find(nm) >= 0) P.fm_active = true;
+}
 
 // Reference-element tables and mesh (buildDOFManagers, getBasis,
 // getQuadrature, SimpleMeshManager-style structured numbering).
@@ -214,6 +239,7 @@ static Problem* build_problem(const mhx_problem_desc* d) {
   P->src.assign(P->ne, 0.0);
   if (d->k0) std::copy(d->k0, d->k0 + P->ne, P->k0.begin());
   if (d->src_elem) std::copy(d->src_elem, d->src_elem + P->ne, P->src.begin());
+  set_functions(*P, *d);
 
   // vertices
   double lo[3] = {d->xmin, d->ymin, d->zmin}, hi[3] = {d->xmax, d->ymax, d->zmax};
@@ -469,24 +495,58 @@ static void assemble_thermal(const Problem& P, const double* u, bool want_jac, d
         }
       }
       // FunctionManager::evaluate("thermal source" / "thermal diffusion" /
-      // "density" / "specific heat")
-      std::vector<double> source(nq);
-      std::vector<EvalT> diff(nq);
-      const double rho = P.desc.th_density > 0.0 ? P.desc.th_density : 1.0;
-      const double cp = P.desc.th_specific_heat > 0.0 ? P.desc.th_specific_heat : 1.0;
-      for (int q = 0; q < nq; ++q) {
-        source[q] = source_at(P, e, &G.x[q * 3]);
-        EvalT TT = T[q] * T[q];
-        EvalT onep = 1.0 + TT * P.desc.diff_u2;
-        diff[q] = P.k0[e] * (P.desc.diff_u2 != 0.0 ? onep : EvalT(1.0));
+      // "density" / "specific heat" / "bx" / "by" / "bz")
+      std::vector<EvalT> source(nq), diff(nq), rhocp(nq), bx, by, bz;
+      bool adv = false;
+      if (P.fm_active) {
+        FElem<EvalT> el;
+        el.nq = nq;
+        el.T = T.data();
+        el.dT = dT.data();
+        el.Tt = P.bdf > 0 ? Tt.data() : nullptr;
+        el.x = G.x.data();
+        el.dim = dim;
+        el.time = P.desc.fn_time;
+        el.k0 = P.k0[e];
+        el.src = P.src[e];
+        auto fn = [&](const char* name, double dflt) {
+          return P.fm.find(name) >= 0 ? P.fm.values<EvalT>(name, el) : std::vector<EvalT>(nq, EvalT(dflt));
+        };
+        diff = fn("thermal diffusion", 1.0);
+        source = fn("thermal source", 0.0);
+        const std::vector<EvalT> rho = fn("density", 1.0), cp = fn("specific heat", 1.0);
+        for (int q = 0; q < nq; ++q) rhocp[q] = rho[q] * cp[q];
+        adv = P.fm.find("advection x") >= 0 || P.fm.find("advection y") >= 0 || P.fm.find("advection z") >= 0;
+        if (adv) {
+          bx = fn("advection x", 0.0);
+          by = fn("advection y", 0.0);
+          bz = fn("advection z", 0.0);
+        }
+      } else {
+        const double rho = P.desc.th_density > 0.0 ? P.desc.th_density : 1.0;
+        const double cp = P.desc.th_specific_heat > 0.0 ? P.desc.th_specific_heat : 1.0;
+        for (int q = 0; q < nq; ++q) {
+          source[q] = EvalT(source_at(P, e, &G.x[q * 3]));
+          EvalT TT = T[q] * T[q];
+          EvalT onep = 1.0 + TT * P.desc.diff_u2;
+          diff[q] = P.k0[e] * (P.desc.diff_u2 != 0.0 ? onep : EvalT(1.0));
+          rhocp[q] = EvalT(rho * cp);
+        }
       }
-      // "Thermal volume resid 3D part 1": (rho cp dT/dt - source) w phi + diff grad T . grad phi w
+      // "Thermal volume resid 3D part 1": (rho cp dT/dt - source) w phi +
+      // diff grad T . grad phi w (+ b . grad T w phi with advection)
       std::vector<EvalT> r(nb, EvalT(0.0));
       for (int i = 0; i < nb; ++i) {
         for (int q = 0; q < nq; ++q) {
-          r[i] += (Tt[q] * (rho * cp) - source[q]) * G.wts[q] * P.bref[i * nq + q];
+          r[i] += (rhocp[q] * Tt[q] - source[q]) * G.wts[q] * P.bref[i * nq + q];
           for (int a = 0; a < dim; ++a)
             r[i] += diff[q] * dT[q * dim + a] * G.wts[q] * G.grad[(i * nq + q) * dim + a];
+          if (adv) {
+            EvalT bg = bx[q] * dT[q * dim];
+            if (dim > 1) bg += by[q] * dT[q * dim + 1];
+            if (dim > 2) bg += bz[q] * dT[q * dim + 2];
+            r[i] += bg * G.wts[q] * P.bref[i * nq + q];
+          }
         }
       }
       // scatter: res -= r.val(); J(row, cols) += r.dx(col); skip fixed rows
@@ -1147,7 +1207,61 @@ int orc_create(const mhx_problem_desc* d, void** out) {
     orc::g_err = "orc_create: density and viscosity must be positive";
     return MHX_ERR_ARG;
   }
-  *out = orc::build_problem(d);
+  try {
+    *out = orc::build_problem(d);
+  } catch (const std::exception& ex) {
+    orc::g_err = ex.what();
+    return MHX_ERR_ARG;
+  }
+  return MHX_OK;
+}
+
+// FunctionManager::evaluate of one function at n points: each point its own
+// element (one_element = 0), or all n points one element (1, for the element
+// reductions).  pts[11 i ..]: T, T_t, grad(T)[x], [y], [z], x, y, z, t, k0,
+// src; out[6 i ..]: value and the derivatives by T, grad(T)[x|y|z], T_t.
+int orc_expr_eval(const mhx_problem_desc* d, const char* name, int64_t n, const double* pts, double* out,
+                  int32_t one_element) {
+  using E = orc::SFad<5>;
+  try {
+    orc::FManager fm;
+    fm.have_k0 = fm.have_src = true;
+    for (int i = 0; i < d->n_functions; ++i) fm.add(d->function_names[i], d->function_exprs[i]);
+    fm.decompose();
+    const int64_t per = one_element ? n : 1;
+    for (int64_t s0 = 0; s0 < n; s0 += per) {
+      std::vector<E> T(per), dT(per * 3), Tt(per);
+      std::vector<double> x(per * 3);
+      for (int64_t q = 0; q < per; ++q) {
+        const double* p = pts + 11 * (s0 + q);
+        T[q] = E(5, 0, p[0]);
+        Tt[q] = E(5, 4, p[1]);
+        for (int c = 0; c < 3; ++c) {
+          dT[q * 3 + c] = E(5, 1 + c, p[2 + c]);
+          x[q * 3 + c] = p[5 + c];
+        }
+      }
+      const double* p0 = pts + 11 * s0;
+      orc::FElem<E> el;
+      el.nq = (int)per;
+      el.T = T.data();
+      el.dT = dT.data();
+      el.Tt = Tt.data();
+      el.x = x.data();
+      el.dim = 3;
+      el.time = p0[8];
+      el.k0 = p0[9];
+      el.src = p0[10];
+      const std::vector<E> v = fm.values<E>(name, el);
+      for (int64_t q = 0; q < per; ++q) {
+        out[6 * (s0 + q)] = v[q].v;
+        for (int k = 0; k < 5; ++k) out[6 * (s0 + q) + 1 + k] = v[q].d[k];
+      }
+    }
+  } catch (const std::exception& ex) {
+    orc::g_err = ex.what();
+    return MHX_ERR_ARG;
+  }
   return MHX_OK;
 }
 void orc_destroy(void* h) { delete (orc::Problem*)h; }
@@ -1223,6 +1337,15 @@ int orc_set_time_state(void* h, double dt, int32_t order, const double* u_prev) 
 // the constant Dirichlet value (the mesh and sides stay)
 int orc_update(void* h, const mhx_problem_desc* d) {
   auto* P = (orc::Problem*)h;
+  try {
+    if (d->k0) std::copy(d->k0, d->k0 + P->ne, P->k0.begin());
+    if (d->src_elem) std::copy(d->src_elem, d->src_elem + P->ne, P->src.begin());
+    orc::set_functions(*P, *d);
+  } catch (const std::exception& ex) {
+    orc::g_err = ex.what();
+    return MHX_ERR_ARG;
+  }
+  P->desc.fn_time = d->fn_time;
   P->desc.n_modes = d->n_modes;
   std::copy(d->mode_amp, d->mode_amp + MHX_MAX_MODES, P->desc.mode_amp);
   for (int m = 0; m < MHX_MAX_MODES; ++m)

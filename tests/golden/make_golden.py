@@ -5,8 +5,9 @@ Reads (from a MrHyDE checkout, default /root/reference):
   regression/thermal/*/mrhyde.gold                L2 error norms of manufactured solutions
   regression/navierstokes/channel/mrhyde.gold     L2 error norms of the channel flow
 and writes tests/golden/hgrad_kat.json, thermal_gold.json and ns_gold.json.
-Only numbers (inputs and expected outputs) are stored; the deck parameters of
-each case are transcribed next to its gold values.
+Only numbers (inputs and expected outputs) are stored, with each deck's
+"Functions" strings (its coefficient expressions, input data); the deck
+parameters of each case are transcribed next to its gold values.
 
 Usage: python tests/golden/make_golden.py [/path/to/MrHyDE]
 """
@@ -107,13 +108,36 @@ def transient_norms(path: Path) -> list:
     return out
 
 
+def deck_functions(deck_dir: Path) -> dict:
+    """The deck's "Functions" block verbatim (name -> expression string), from
+    input.yaml or, where a directory holds several decks, the first input*.yaml."""
+    import yaml
+
+    files = [deck_dir / "input.yaml"] if (deck_dir / "input.yaml").exists() else sorted(deck_dir.glob("input*.yaml"))
+    if not files:
+        return {}
+    doc = yaml.safe_load(files[0].read_text())
+
+    def find(o):
+        if isinstance(o, dict):
+            for k, v in o.items():
+                if k == "Functions" and isinstance(v, dict):
+                    return {str(a): str(b) for a, b in v.items() if not isinstance(b, dict)}
+                r = find(v)
+                if r is not None:
+                    return r
+        return None
+
+    return find(doc) or {}
+
+
 def main(ref: Path):
     kat = parse_hgrad(ref / "regression/discretization/HGRAD/mrhyde.gold")
     (HERE / "hgrad_kat.json").write_text(json.dumps(kat, indent=1))
     gold = {}
     for name, deck in CASES.items():
         g = norms(ref / "regression/thermal" / name / "mrhyde.gold")
-        gold[name] = dict(deck, gold=g)
+        gold[name] = dict(deck, functions=deck_functions(ref / "regression/thermal" / name), gold=g)
     gold["_note"] = ("amplitudes of 'source' are in units of pi^2 (source = amp*pi^2*prod sin(w*pi*x)); "
                      "'true' amplitudes are plain; frequencies are in units of pi")
     (HERE / "thermal_gold.json").write_text(json.dumps(gold, indent=1))
@@ -125,7 +149,8 @@ def main(ref: Path):
     (HERE / "ns_gold.json").write_text(json.dumps(ns, indent=1))
     tr = {}
     for name, deck in TRANSIENT_CASES.items():
-        tr[name] = dict(deck, gold=transient_norms(ref / "regression/thermal" / name / "mrhyde.gold"))
+        tr[name] = dict(deck, functions=deck_functions(ref / "regression/thermal" / name),
+                        gold=transient_norms(ref / "regression/thermal" / name / "mrhyde.gold"))
     tr["_note"] = "gold: [time, L2 norm of the error for T] per step"
     (HERE / "transient_gold.json").write_text(json.dumps(tr, indent=1))
     print("wrote", HERE / "hgrad_kat.json", HERE / "thermal_gold.json", HERE / "ns_gold.json",
