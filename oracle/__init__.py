@@ -47,6 +47,7 @@ def _load():
         "orc_set_time_state": (C.c_int, [vp, C.c_double, i32, vp]),
         "orc_update": (C.c_int, [vp, C.POINTER(ProblemDesc)]),
         "orc_expr_eval": (C.c_int, [C.POINTER(ProblemDesc), C.c_char_p, i64, vp, vp, i32]),
+        "orc_set_threads": (C.c_int, [vp, i32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -156,6 +157,11 @@ class Oracle:
         rc = lib.orc_set_time_state(self._h, float(dt), int(order), buf.ctypes.data)
         if rc != 0:
             raise ValueError("orc_set_time_state: bad arguments")
+
+    def set_threads(self, n: int) -> int:
+        """OpenMP threads of the thermal assembly and of the Krylov vector
+        work (1, the default, keeps the sequential order of every sum)."""
+        return lib.orc_set_threads(self._h, int(n))
 
     def update(self, problem: Problem):
         """New source amplitudes / density / specific heat / Dirichlet constant."""

@@ -163,6 +163,7 @@ struct Problem {
   int nel[3], nn[3], nvx[3];         // elements, DOF nodes, vertices per axis
   long ne, ndof, nvert, nnode;       // ndof = nnode * nf, DOF = node * nf + field
   std::vector<double> k0, src;       // per element
+  int threads = 1;                   // OpenMP threads of the thermal assembly (1: sequential order)
   FManager fm;                       // coefficient functions (FunctionManager)
   bool fm_active = false;            // a thermal coefficient is given as a function
   std::vector<double> vx;            // vertex coords [nvert*3]
@@ -456,13 +457,24 @@ static inline long find_slot(const Problem& P, int row, int col) {
 // residual of thermal.cpp:71-165 (steady, no advection) and the scatter of
 // assemblyManager_scatter.hpp:164-278; then dofConstraints
 // (assemblyManager_constraints.hpp:241-270).
+// With P.threads > 1 the worksets run on that many OpenMP threads and the
+// scatter adds atomically, as the reference's Kokkos-OpenMP build does
+// (assemblyManager_scatter.hpp:227, 261: Kokkos::atomic_add); the sums'
+// order then varies from run to run.  One thread (the default) keeps the
+// sequential order.
 template <class EvalT>
 static void assemble_thermal(const Problem& P, const double* u, bool want_jac, double* res, double* jv) {
-  int ws = P.desc.workset_size > 0 ? P.desc.workset_size : 100;
-  int nb = P.nb, nq = P.nq, dim = P.dim;
+  const int ws = P.desc.workset_size > 0 ? P.desc.workset_size : 100;
+  const int nb = P.nb, nq = P.nq, dim = P.dim;
+  const long nws = (P.ne + ws - 1) / ws;
+  const bool par = P.threads > 1;
+#pragma omp parallel num_threads(P.threads) if (par)
+  {
   std::vector<EvalT> uAD, T, dT;
   ElemGeom G;
-  for (long e0 = 0; e0 < P.ne; e0 += ws) {
+#pragma omp for schedule(dynamic, 1)
+  for (long wsi = 0; wsi < nws; ++wsi) {
+    const long e0 = wsi * ws;
     long nel = std::min<long>(ws, P.ne - e0);
     uAD.assign(nel * nb, EvalT(0.0));
     // performGather + computeSolnSteadySeeded
@@ -554,12 +566,30 @@ static void assemble_thermal(const Problem& P, const double* u, bool want_jac, d
       for (int i = 0; i < nb; ++i) {
         int row = L[i];
         if (P.fixed[row]) continue;
-        if (res) res[row] += -val(r[i]);
+        if (res) {
+          const double v = -val(r[i]);
+          if (par) {
+#pragma omp atomic
+            res[row] += v;
+          } else {
+            res[row] += v;
+          }
+        }
         if (want_jac && jv)
-          for (int j = 0; j < nb; ++j) jv[find_slot(P, row, L[j])] += dx(r[i], j);
+          for (int j = 0; j < nb; ++j) {
+            const double v = dx(r[i], j);
+            double* slot = &jv[find_slot(P, row, L[j])];
+            if (par) {
+#pragma omp atomic
+              *slot += v;
+            } else {
+              *slot += v;
+            }
+          }
       }
     }
   }
+  }  // omp parallel
   if (want_jac && jv)
     for (long g = 0; g < P.ndof; ++g)
       if (P.fixed[g]) jv[find_slot(P, (int)g, (int)g)] = 1.0;
@@ -985,13 +1015,21 @@ static void assemble(const Problem& P, const double* u, bool want_jac, double* r
 // Krylov
 // ---------------------------------------------------------------------------
 static void matvec(const Problem& P, const double* A, const double* x, double* y) {
+#pragma omp parallel for num_threads(P.threads) if (P.threads > 1) schedule(static)
   for (long r = 0; r < P.ndof; ++r) {
     double s = 0.0;
     for (int k = P.rowptr[r]; k < P.rowptr[r + 1]; ++k) s += A[k] * x[P.colidx[k]];
     y[r] = s;
   }
 }
-static double dot(long n, const double* a, const double* b) { double s = 0; for (long i = 0; i < n; ++i) s += a[i] * b[i]; return s; }
+// (with more than one thread the sum's order follows the thread split)
+static int g_dot_threads = 1;
+static double dot(long n, const double* a, const double* b) {
+  double s = 0;
+#pragma omp parallel for num_threads(g_dot_threads) if (g_dot_threads > 1) reduction(+ : s) schedule(static)
+  for (long i = 0; i < n; ++i) s += a[i] * b[i];
+  return s;
+}
 
 // SolverManager::PCG (solverManager_solvers.hpp:812-905): Jacobi PCG,
 // stopping on ||r||/||r0|| <= tol with r0 = b - A x0.
@@ -1049,16 +1087,24 @@ static int gmres(const Problem& P, const double* A, const double* b, double* x, 
     g[0] = beta;
     int k = 0;
     for (; k < m && it < maxit; ++k, ++it) {
+      const int nt = P.threads;
+#pragma omp parallel for num_threads(nt) if (nt > 1) schedule(static)
       for (long i = 0; i < n; ++i) tmp[i] = Minv[i] * V[k][i];
       matvec(P, A, tmp.data(), w.data());
       for (int j = 0; j <= k; ++j) {
         double h = dot(n, w.data(), V[j].data());
         H[j * m + k] = h;
-        for (long i = 0; i < n; ++i) w[i] -= h * V[j][i];
+        const double* vj = V[j].data();
+#pragma omp parallel for num_threads(nt) if (nt > 1) schedule(static)
+        for (long i = 0; i < n; ++i) w[i] -= h * vj[i];
       }
       double hn = std::sqrt(dot(n, w.data(), w.data()));
       H[(k + 1) * m + k] = hn;
-      if (hn != 0.0) for (long i = 0; i < n; ++i) V[k + 1][i] = w[i] / hn;
+      double* vk1 = V[k + 1].data();
+      if (hn != 0.0) {
+#pragma omp parallel for num_threads(nt) if (nt > 1) schedule(static)
+        for (long i = 0; i < n; ++i) vk1[i] = w[i] / hn;
+      }
       for (int j = 0; j < k; ++j) {
         double a = H[j * m + k], bb = H[(j + 1) * m + k];
         H[j * m + k] = cs[j] * a + sn[j] * bb;
@@ -1091,6 +1137,7 @@ static int gmres(const Problem& P, const double* A, const double* b, double* x, 
 
 static int linear_solve(const Problem& P, const double* A, const double* b, double* x, const mhx_krylov_opts* o,
                         mhx_solve_stats* st) {
+  g_dot_threads = P.threads;
   double rel = 0.0;
   int it;
   if (o->solver == MHX_KRYLOV_CG) it = pcg(P, A, b, x, o->tol, o->max_iters, &rel);
@@ -1294,6 +1341,12 @@ int orc_lids(void* h, int32_t* lids) {
 }
 // res/jv are accumulated into (the caller zeroes them, as
 // solverManager_solvers.hpp:347-354 does).
+// threads of the thermal assembly (the OpenMP baseline); returns the count used
+int orc_set_threads(void* h, int32_t n) {
+  auto* P = (orc::Problem*)h;
+  P->threads = n < 1 ? 1 : n;
+  return P->threads;
+}
 int orc_assemble(void* h, const double* u, int32_t want_jac, double* res, double* jv) {
   orc::assemble(*(orc::Problem*)h, u, want_jac != 0, res, jv);
   return MHX_OK;
