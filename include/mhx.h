@@ -321,6 +321,24 @@ int  mhx_amg_level(mhx_ctx* ctx, int32_t level, int32_t which, int64_t info[4], 
  * mhx_ctx_update).  bdf_order = 0 returns to steady assembly. */
 int  mhx_set_time_state(mhx_ctx* ctx, double dt, int32_t bdf_order, const double* u_prev, int32_t flags);
 
+/* One stage of a diagonally implicit Runge-Kutta step (SolverManager::
+ * transientSolver's stage loop, src/managers/solver/solverManager_solvers.hpp
+ * :125-151, with Workset::computeSolnTransientSeeded, src/tools/workset.cpp
+ * :601-680).  The following assemblies and Newton solves take the unknown U
+ * as stage `stage` of the tableau (A row-major n_stages x n_stages, b):
+ *   T      = alpha_u U + (1 - alpha_u) u_prev[0]
+ *            + sum_{j<stage} A_sj / b_j (u_stages[j] - u_prev[0]),
+ *   dT/dt  = (BDF_0 U + sum_{k>=1} BDF_k u_prev[k-1]) / (dt b_stage),
+ * alpha_u = A_ss / b_s.  After the last stage the step's solution is
+ * u_prev[0] + sum_s (U_s - u_prev[0]) (solverManager_solvers.hpp:145-150);
+ * coefficients at the stage time t_n + c_s dt come from mhx_ctx_update.
+ * u_stages: `stage` vectors (MHX_DEVICE_PTRS: device pointers).  Weak
+ * Dirichlet sides are refused for multi-stage tableaus.  mhx_set_time_state
+ * is this call with the one-stage tableau A = b = 1. */
+int  mhx_set_stage_state(mhx_ctx* ctx, double dt, int32_t bdf_order, const double* u_prev, int32_t n_stages,
+                         const double* butcher_A, const double* butcher_b, int32_t stage, const double* u_stages,
+                         int32_t flags);
+
 /* Newton on u in place (u must already carry the Dirichlet values, see
  * mhx_set_dirichlet).  Replaces SolverManager::nonlinearSolver
  * (src/managers/solver/solverManager_solvers.hpp:290-643); status 0 on

@@ -121,6 +121,40 @@ def bdf_schedule(order: int, nsteps: int, startup_order=None, startup_steps=None
     return [so if (order > 1 and step < ss) else order for step in range(nsteps)]
 
 
+def butcher_tableau(name: str, A=None, b=None, c=None):
+    """Butcher tableau (A, b, c) by the reference's "transient Butcher
+    tableau" name (SolverManager::setButcherTableau,
+    solverManager_setup.hpp:521-700): the diagonally implicit ones and CN;
+    "custom" takes A, b, c.  The explicit tableaus (FWE, SSPRK-3,3, RK-4,4,
+    leap-frog) run on the reference's separate explicit solver and are not
+    built here."""
+    r3 = np.sqrt(3.0)
+    p = 0.4358665215
+    tab = {
+        "BWE": ([[1.0]], [1.0], [1.0]),
+        "DIRK-1,1": ([[1.0]], [1.0], [1.0]),
+        "CN": ([[0.0, 0.0], [0.5, 0.5]], [0.5, 0.5], [0.0, 1.0]),
+        "DIRK-1,2": ([[0.5]], [1.0], [0.5]),
+        "DIRK-2,2": ([[0.25, 0.0], [0.5, 0.25]], [0.5, 0.5], [0.25, 0.75]),
+        "DIRK-2,3": ([[0.5 + r3 / 6.0, 0.0], [-r3 / 3.0, 0.5 + r3 / 6.0]], [0.5, 0.5],
+                     [0.5 + r3 / 6.0, 0.5 - r3 / 6.0]),
+        "DIRK-3,3": ([[p, 0.0, 0.0], [(1.0 - p) / 2.0, p, 0.0],
+                      [-3.0 * p * p / 2.0 + 4.0 * p - 1.0 / 4.0, 3.0 * p * p / 2.0 - 5.0 * p + 5.0 / 4.0, p]],
+                     [-3.0 * p * p / 2.0 + 4.0 * p - 1.0 / 4.0, 3.0 * p * p / 2.0 - 5.0 * p + 5.0 / 4.0, p],
+                     [p, (1.0 + p) / 2.0, 1.0]),
+    }
+    if name == "custom":
+        A, b, c = np.atleast_2d(np.asarray(A, float)), np.asarray(b, float).ravel(), np.asarray(c, float).ravel()
+    elif name in tab:
+        A, b, c = (np.asarray(v, float) for v in tab[name])
+    else:
+        raise ValueError(f"Butcher tableau {name!r} is not built (explicit tableaus run on the reference's "
+                         "explicit solver)")
+    if A.shape != (len(b), len(b)) or len(c) != len(b) or np.any(np.triu(A, 1) != 0.0):
+        raise ValueError("Butcher tableau: A must be square, lower triangular, and match b and c")
+    return np.ascontiguousarray(A), b, c
+
+
 def subgrid_workload(n_inst: int = 4096, seed: int = 4096, sizes=(4, 6, 8, 12, 16), sigma: float = 1.0):
     """The batched-subgrid input set: per instance n ~ U(sizes), element
     conductivity exp(sigma z), z ~ N(0,1), and Dirichlet data U[-1,1] on the

@@ -64,6 +64,7 @@ def _load() -> C.CDLL:
         "mhx_server_destroy": (None, [vp]),
         "mhx_amg_setup": (C.c_int, [vp, vp, i32]),
         "mhx_set_time_state": (C.c_int, [vp, C.c_double, i32, vp, i32]),
+        "mhx_set_stage_state": (C.c_int, [vp, C.c_double, i32, vp, i32, vp, vp, i32, vp, i32]),
         "mhx_write_matrix_market": (C.c_int, [vp, C.c_char_p, vp, i32]),
         "mhx_write_vector_market": (C.c_int, [vp, C.c_char_p, vp, i32]),
         "mhx_read_matrix_market": (C.c_int, [vp, C.c_char_p, vp, i32]),
@@ -92,4 +93,5 @@ EXPORTED = [
     "mhx_server_info", "mhx_server_last_error", "mhx_server_destroy", "mhx_amg_setup", "mhx_amg_level",
     "mhx_set_time_state", "mhx_write_matrix_market", "mhx_write_vector_market", "mhx_read_matrix_market",
     "mhx_read_vector_market", "mhx_linear_solve_noalloc", "mhx_expr_eval", "mhx_expr_last_error",
+    "mhx_set_stage_state",
 ]

@@ -169,6 +169,8 @@ void assemble_thermal(Ctx& c, const double* u, double* res, double* jac, bool wa
   if (c.bdf_order > 0) {
     a.hist = c.hist.p;
     a.alpha_t = c.alpha_t;
+    a.hist_u = c.hist_u.p;
+    a.alpha_u = c.alpha_u;
     const double rho = c.desc.th_density > 0.0 ? c.desc.th_density : 1.0;
     const double cp = c.desc.th_specific_heat > 0.0 ? c.desc.th_specific_heat : 1.0;
     a.rho_cp = rho * cp;

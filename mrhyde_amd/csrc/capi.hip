@@ -569,19 +569,54 @@ int mhx_linear_solve_noalloc(mhx_ctx* ctx, const double* jac_vals, const double*
   MHX_GUARD_END(ctx)
 }
 
-int mhx_set_time_state(mhx_ctx* ctx, double dt, int32_t bdf_order, const double* u_prev, int32_t flags) {
-  MHX_GUARD_BEGIN
-  MHX_REQUIRE(ctx, MHX_ERR_ARG, "null argument");
-  Ctx& c = *ctx;
+// nodal history terms of a time step / Butcher stage
+// (Workset::computeSolnTransientSeeded, workset.cpp:601-680):
+//   beta_t = (sum_{k>=1} BDF_k u_prev[k-1]) * timewt
+//   beta_u = (1 - alpha_u) u_prev[0] + sum_{j<s} A_sj / b_j (U_j - u_prev[0])
+// in the reference's order of operations
+constexpr int kMaxStages = 8;
+struct StageCoef {
+  double w[7];            // BDF_1..BDF_k
+  double a[kMaxStages];   // A_sj / b_j, j < s
+};
+__global__ void k_stage_history(long n, int order, const double* __restrict__ prev, int stage,
+                                const double* __restrict__ ustage, double timewt, double alpha_u, StageCoef sc,
+                                double* __restrict__ beta_t, double* __restrict__ beta_u) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double bt = 0.0;
+  for (int k = 1; k <= order; ++k) bt += sc.w[k - 1] * prev[(long)(k - 1) * n + i];
+  beta_t[i] = bt * timewt;
+  if (beta_u) {
+    const double u0 = prev[i];
+    double bu = (1.0 - alpha_u) * u0;
+    for (int j = 0; j < stage; ++j) bu += sc.a[j] * (ustage[(long)j * n + i] - u0);
+    beta_u[i] = bu;
+  }
+}
+
+static void set_stage_state(Ctx& c, double dt, int32_t bdf_order, const double* u_prev, int32_t n_stages,
+                            const double* A, const double* b, int32_t stage, const double* u_stages, int32_t flags) {
   MHX_REQUIRE(bdf_order >= 0 && bdf_order <= 6, MHX_ERR_ARG, "bdf_order must be 0..6");
   if (bdf_order == 0) {
     c.bdf_order = 0;
     c.alpha_t = 0.0;
-    return MHX_OK;
+    c.alpha_u = 1.0;
+    c.hist_u.release();
+    return;
   }
   MHX_REQUIRE(c.desc.physics == MHX_PHYS_THERMAL, MHX_ERR_UNSUPPORTED, "time integration is built for the thermal module");
   MHX_REQUIRE(dt > 0.0 && std::isfinite(dt), MHX_ERR_ARG, "dt must be positive");
   MHX_REQUIRE(u_prev, MHX_ERR_ARG, "null history");
+  MHX_REQUIRE(n_stages >= 1 && n_stages <= kMaxStages && stage >= 0 && stage < n_stages && A && b, MHX_ERR_ARG,
+              "Butcher tableau: 1..8 stages, stage in range, A and b given");
+  MHX_REQUIRE(stage == 0 || u_stages, MHX_ERR_ARG, "the earlier stages' solutions are needed");
+  const double bs = b[stage], ass = A[stage * n_stages + stage];
+  MHX_REQUIRE(bs != 0.0 && std::isfinite(bs) && std::isfinite(ass), MHX_ERR_ARG, "Butcher b of the stage must be nonzero");
+  for (int j = 0; j < stage; ++j) MHX_REQUIRE(b[j] != 0.0, MHX_ERR_ARG, "Butcher b of an earlier stage is zero");
+  const bool multi = n_stages > 1 || ass != 1.0 || bs != 1.0;
+  MHX_REQUIRE(!multi || !c.desc.wdbc_sides, MHX_ERR_UNSUPPORTED,
+              "weak Dirichlet sides take the stage unknown; built for one-stage tableaus only");
   // backward-difference weights without 1/dt (solverManager_setup.hpp:455-500)
   static const double W[7][7] = {
       {0}, {1.0, -1.0}, {1.5, -2.0, 0.5}, {11.0 / 6.0, -3.0, 1.5, -1.0 / 3.0},
@@ -589,23 +624,55 @@ int mhx_set_time_state(mhx_ctx* ctx, double dt, int32_t bdf_order, const double*
       {137.0 / 60.0, -5.0, 5.0, -10.0 / 3.0, 75.0 / 60.0, -1.0 / 5.0},
       {147.0 / 60.0, -6.0, 15.0 / 2.0, -20.0 / 3.0, 225.0 / 60.0, -72.0 / 60.0, 1.0 / 6.0}};
   const long n = c.mesh.ndof;
+  const bool dev = (flags & MHX_DEVICE_PTRS) != 0;
   const double* prev = u_prev;
-  DevBuf<double> stage;
-  if (!(flags & MHX_DEVICE_PTRS)) {
-    stage.alloc((size_t)n * bdf_order);
-    MHX_HIP(hipMemcpyAsync(stage.p, u_prev, (size_t)n * bdf_order * sizeof(double), hipMemcpyHostToDevice,
+  const double* ust = u_stages;
+  DevBuf<double> stg_prev, stg_st;
+  if (!dev) {
+    stg_prev.alloc((size_t)n * bdf_order);
+    MHX_HIP(hipMemcpyAsync(stg_prev.p, u_prev, (size_t)n * bdf_order * sizeof(double), hipMemcpyHostToDevice,
                            c.stream));
-    prev = stage.p;
+    prev = stg_prev.p;
+    if (stage > 0) {
+      stg_st.alloc((size_t)n * stage);
+      MHX_HIP(hipMemcpyAsync(stg_st.p, u_stages, (size_t)n * stage * sizeof(double), hipMemcpyHostToDevice,
+                             c.stream));
+      ust = stg_st.p;
+    }
   }
+  const double timewt = 1.0 / dt / bs;
+  const double alpha_u = ass / bs;
+  StageCoef sc{};
+  for (int k = 1; k <= bdf_order; ++k) sc.w[k - 1] = W[bdf_order][k];
+  for (int j = 0; j < stage; ++j) sc.a[j] = A[stage * n_stages + j] / b[j];
   c.hist.alloc(n);
-  MHX_HIP(hipMemsetAsync(c.hist.p, 0, n * sizeof(double), c.stream));
-  // hist = sum_k BDF_k u_prev[k-1] / dt, summed in k order
-  const double timewt = 1.0 / dt;
-  for (int k = 1; k <= bdf_order; ++k) axpy(c, n, W[bdf_order][k] * timewt, prev + (long)(k - 1) * n, c.hist.p);
+  if (multi) c.hist_u.alloc(n);
+  else c.hist_u.release();
+  hipLaunchKernelGGL(k_stage_history, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream, n, (int)bdf_order,
+                     prev, (int)stage, ust, timewt, alpha_u, sc, c.hist.p, multi ? c.hist_u.p : nullptr);
+  MHX_HIP(hipGetLastError());
   c.alpha_t = W[bdf_order][0] * timewt;
+  c.alpha_u = multi ? alpha_u : 1.0;
   c.bdf_order = bdf_order;
   c.jac_valid = false;
   MHX_HIP(hipStreamSynchronize(c.stream));
+}
+
+int mhx_set_time_state(mhx_ctx* ctx, double dt, int32_t bdf_order, const double* u_prev, int32_t flags) {
+  MHX_GUARD_BEGIN
+  MHX_REQUIRE(ctx, MHX_ERR_ARG, "null argument");
+  const double one = 1.0;
+  set_stage_state(*ctx, dt, bdf_order, u_prev, 1, &one, &one, 0, nullptr, flags);
+  return MHX_OK;
+  MHX_GUARD_END(ctx)
+}
+
+int mhx_set_stage_state(mhx_ctx* ctx, double dt, int32_t bdf_order, const double* u_prev, int32_t n_stages,
+                        const double* butcher_A, const double* butcher_b, int32_t stage, const double* u_stages,
+                        int32_t flags) {
+  MHX_GUARD_BEGIN
+  MHX_REQUIRE(ctx, MHX_ERR_ARG, "null argument");
+  set_stage_state(*ctx, dt, bdf_order, u_prev, n_stages, butcher_A, butcher_b, stage, u_stages, flags);
   return MHX_OK;
   MHX_GUARD_END(ctx)
 }

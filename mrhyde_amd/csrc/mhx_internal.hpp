@@ -91,8 +91,10 @@ struct AsmArgs {
   int bw[3];              // block colouring: block width in elements (2*s)
   int first_touch;        // overwrite on first touch instead of accumulate
   // transient (backward differences): dT/dt = alpha_t u + sum_j hist_j phi_j
-  const double* hist;     // [ndof] nodal sum_k BDF_k u_prev[k-1] / dt, or nullptr (steady)
-  double alpha_t;         // BDF_0 / dt
+  const double* hist;     // [ndof] nodal beta_t = sum_k BDF_k u_prev[k-1] / (dt b_s), or nullptr (steady)
+  double alpha_t;         // BDF_0 / (dt b_s)
+  const double* hist_u;   // [ndof] nodal beta_u of a Butcher stage, or nullptr (alpha_u = 1, beta_u = 0)
+  double alpha_u;         // A_ss / b_s
   double rho_cp;          // density * specific heat
   ExprSet xp;             // compiled coefficient functions (xp.active = 0: the closed forms)
 };
@@ -238,6 +240,8 @@ struct Ctx {
   int bdf_order = 0;
   double alpha_t = 0.0;
   DevBuf<double> hist;       // [ndof] nodal history term of dT/dt
+  DevBuf<double> hist_u;     // [ndof] nodal beta_u of a Butcher stage (released for one-stage tableaus)
+  double alpha_u = 1.0;
   // compiled coefficient functions
   ExprHost xh;
   DevBuf<ExprIns> x_code;

@@ -130,9 +130,11 @@ __device__ __forceinline__ void row_range(int c, int p, int n, int& lo, int& len
   }
 }
 
-// TR: transient, the source row gains rho cp (alpha_t u + h(x)) with h the
-// nodal history term (workset.cpp:601-680 seeds u_dot = alpha_t u + beta_t
-// per node; thermal.cpp:129 multiplies it by rho cp)
+// TR: transient (workset.cpp:601-680): per node the evaluated solution is
+// alpha_u U + beta_u and its time derivative alpha_t U + beta_t, U the stage
+// unknown (alpha_u = 1, beta_u = 0 for a single-stage tableau); the source
+// row gains rho cp dT/dt (thermal.cpp:129) and every derivative slot the
+// factor of its seed
 // EX: the coefficients come from compiled expressions (a.xp) instead of the
 // closed forms (element colouring only)
 template <int DIM, int P, int NQ1, bool JAC, int MODE, bool TR = false, bool EX = false>
@@ -187,8 +189,13 @@ __global__ __launch_bounds__(256) void k_thermal(AsmArgs a) {
   if (active) {
     for (int j = lane; j < NB; j += G) {
       const long nd = node_of<DIM, P>(a.m, ec, j);
-      s_u[g][j] = a.u[nd];
-      if constexpr (TR) s_h[g][j] = a.hist[nd];
+      if constexpr (TR) {
+        const double un = a.u[nd];
+        s_u[g][j] = a.hist_u ? fma(a.alpha_u, un, a.hist_u[nd]) : un;
+        s_h[g][j] = fma(a.alpha_t, un, a.hist[nd]);
+      } else {
+        s_u[g][j] = a.u[nd];
+      }
     }
     for (int t = lane; t < NV * DIM; t += G) {
       int v = t / DIM, d = t % DIM;
@@ -316,6 +323,12 @@ __global__ __launch_bounds__(256) void k_thermal(AsmArgs a) {
       QDual<NC> Gd[DIM];
 #pragma unroll
       for (int c = 0; c < DIM; ++c) Gd[c] = qd_seed<NC>(gp[c], c);
+      if constexpr (TR) {
+        // d(alpha_u U + beta_u)/dU
+        U.d[DIM] = a.alpha_u;
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) Gd[c].d[c] = a.alpha_u;
+      }
       QDual<NC> flux[DIM], src;
       if constexpr (EX) {
         // thermal.cpp:80-92, 125-131: diff, source, cp, rho, advection from
@@ -329,7 +342,7 @@ __global__ __launch_bounds__(256) void k_thermal(AsmArgs a) {
         }
         pt.Tt = xq_const<NC>(0.0);
         if constexpr (TR) {
-          pt.Tt.v = fma(a.alpha_t, uq, hq);
+          pt.Tt.v = hq;
           pt.Tt.d[DIM] = a.alpha_t;
         }
         pt.time = a.xp.time;
@@ -355,8 +368,8 @@ __global__ __launch_bounds__(256) void k_thermal(AsmArgs a) {
         const double f = thermal_source(a.coef, e, x, DIM);
         ThermalPhysics::volume<DIM>(k0, a.coef.diff_u2, f, U, Gd, flux, src);
         if constexpr (TR) {
-          // + rho cp dT/dt, dT/dt = alpha_t U + h(x): value and d/du slot
-          src.v += a.rho_cp * fma(a.alpha_t, uq, hq);
+          // + rho cp dT/dt, dT/dt interpolated from alpha_t U + beta_t: value and d/dU slot
+          src.v += a.rho_cp * hq;
           src.d[DIM] += a.rho_cp * a.alpha_t;
         }
       }

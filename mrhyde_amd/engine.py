@@ -21,7 +21,7 @@ from typing import Optional
 import numpy as np
 
 from . import capi
-from .abi import Problem, bdf_schedule, rng_fill  # noqa: F401
+from .abi import Problem, bdf_schedule, butcher_tableau, rng_fill  # noqa: F401
 from .capi import lib
 
 
@@ -364,28 +364,57 @@ class Mhx:
             self._check(lib.mhx_set_time_state(self._h, float(dt), int(order), C.c_void_p(buf.data_ptr()),
                                                capi.MHX_DEVICE_PTRS), "set_time_state")
 
+    def set_stage_state(self, dt: float, order: int, u_prev, A, b, stage: int, u_stages=()):
+        """Stage `stage` of the Butcher tableau (A, b) for the following
+        assemblies and solves (mhx_set_stage_state): u_prev as in
+        set_time_state, u_stages the earlier stages' solutions."""
+        A = np.ascontiguousarray(A, dtype=np.float64)
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        prev = np.ascontiguousarray(np.concatenate([np.asarray(v, dtype=np.float64) for v in u_prev[:order]]))
+        st = (np.ascontiguousarray(np.concatenate([np.asarray(v, dtype=np.float64) for v in u_stages[:stage]]))
+              if stage > 0 else None)
+        self._check(lib.mhx_set_stage_state(self._h, float(dt), int(order), prev.ctypes.data, len(b), A.ctypes.data,
+                                            b.ctypes.data, int(stage), st.ctypes.data if st is not None else None, 0),
+                    "set_stage_state")
+
     def transientSolver(self, u0, dt: float, nsteps: int, order: int = 1, startup_order=None, startup_steps=None,
-                        t0: float = 0.0, at_time=None, record=None, **newton_kw):
-        """SolverManager::transientSolver (solverManager_solvers.hpp:40-170)
-        with the BWE tableau: history vectors start as zeros and shift each
-        step (u_prev[0] = u_n); the first `startup_steps` steps use
-        `startup_order` (bdf_schedule); at_time(t) -> Problem gives the
-        coefficients at the stage time t_n + dt; record(t, u) sees each step.
+                        t0: float = 0.0, at_time=None, record=None, tableau: str = "BWE", butcher=None,
+                        **newton_kw):
+        """SolverManager::transientSolver (solverManager_solvers.hpp:40-170):
+        history vectors start as zeros and shift each step (u_prev[0] = u_n);
+        the first `startup_steps` steps use `startup_order` (bdf_schedule);
+        each step runs the stages of the Butcher tableau `tableau` (or
+        butcher = (A, b, c) for "custom"), every stage a Newton solve for its
+        unknown from u_n at the stage time t_n + c_s dt, and the step's
+        solution is u_n + sum_s (U_s - u_n) (:125-151); at_time(t) -> Problem
+        gives the coefficients at a stage time; record(t, u) sees each step.
         Keyword arguments go to nonlinearSolver (defaults: CG + Jacobi to
         1e-12, Newton to 1e-12)."""
+        A, b, c = butcher_tableau(tableau, *(butcher or (None, None, None)))
         kw = dict(max_iters=10, nl_tol=1e-12, solver="cg", tol=1e-12, linear_max_iters=10000)
         kw.update(newton_kw)
         u = np.ascontiguousarray(u0, dtype=np.float64).copy()
         hist = [np.zeros_like(u) for _ in range(max(order, startup_order or order))]
         t = t0
+        one_stage = len(b) == 1 and A[0, 0] == 1.0 and b[0] == 1.0
         for k in bdf_schedule(order, nsteps, startup_order, startup_steps):
             hist = [u.copy()] + hist[:-1]
+            un = u.copy()
+            stages = []
+            for s in range(len(b)):
+                if at_time is not None:
+                    self.update(at_time(t + c[s] * dt))
+                if one_stage:
+                    self.set_time_state(dt, k, hist)
+                else:
+                    self.set_stage_state(dt, k, hist, A, b, s, stages)
+                us = self.setDirichlet(un.copy())
+                us, _ = self.nonlinearSolver(us, **kw)
+                stages.append(us)
+            u = un.copy()
+            for us in stages:
+                u = u + us - un if not one_stage else us
             t = t + dt
-            if at_time is not None:
-                self.update(at_time(t))
-            self.set_time_state(dt, k, hist)
-            self.setDirichlet(u)
-            u, _ = self.nonlinearSolver(u, **kw)
             if record is not None:
                 record(t, u)
         self.set_time_state(dt, 0)

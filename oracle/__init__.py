@@ -48,6 +48,7 @@ def _load():
         "orc_update": (C.c_int, [vp, C.POINTER(ProblemDesc)]),
         "orc_expr_eval": (C.c_int, [C.POINTER(ProblemDesc), C.c_char_p, i64, vp, vp, i32]),
         "orc_set_threads": (C.c_int, [vp, i32]),
+        "orc_set_stage_state": (C.c_int, [vp, C.c_double, i32, vp, i32, vp, vp, i32, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -169,31 +170,60 @@ class Oracle:
         self._desc = problem.desc()
         lib.orc_update(self._h, C.byref(self._desc))
 
+    def set_stage_state(self, dt: float, order: int, u_prev, A, b, stage: int, u_stages=()):
+        """Stage `stage` of the Butcher tableau (A, b) (u_stages: the earlier
+        stages' solutions)."""
+        A = np.ascontiguousarray(A, dtype=np.float64)
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        prev = np.ascontiguousarray(np.concatenate([np.asarray(v, dtype=np.float64) for v in u_prev[:order]]))
+        st = (np.ascontiguousarray(np.concatenate([np.asarray(v, dtype=np.float64) for v in u_stages[:stage]]))
+              if stage > 0 else np.zeros(1))
+        rc = lib.orc_set_stage_state(self._h, float(dt), int(order), prev.ctypes.data, len(b), A.ctypes.data,
+                                     b.ctypes.data, int(stage), st.ctypes.data)
+        if rc != 0:
+            raise ValueError("orc_set_stage_state: bad arguments")
+
     def transientSolver(self, u0, dt: float, nsteps: int, order: int = 1, startup_order=None, startup_steps=None,
-                        t0: float = 0.0, at_time=None, opts: NewtonOpts = None, record=None):
-        """SolverManager::transientSolver (solverManager_solvers.hpp:40-170)
-        with the BWE tableau: history vectors start as zeros and shift each
-        step (u_prev[0] = u_n), the first `startup_steps` steps use
-        `startup_order`; at_time(t) -> Problem gives the coefficients at the
-        stage time t_n + dt, and record(t, u) sees each converged step."""
-        from mrhyde_amd.abi import MHX_KRYLOV_CG, MHX_PREC_JACOBI, bdf_schedule
+                        t0: float = 0.0, at_time=None, opts: NewtonOpts = None, record=None, tableau: str = "BWE",
+                        butcher=None):
+        """SolverManager::transientSolver (solverManager_solvers.hpp:40-170):
+        history vectors start as zeros and shift each step (u_prev[0] = u_n),
+        the first `startup_steps` steps use `startup_order`; each step runs
+        the stages of the Butcher tableau, every stage a Newton solve from u_n
+        at the stage time t_n + c_s dt, and the step's solution is
+        u_n + sum_s (U_s - u_n) (:125-151); at_time(t) -> Problem gives the
+        coefficients at a stage time, and record(t, u) sees each step."""
+        from mrhyde_amd.abi import MHX_KRYLOV_CG, MHX_PREC_JACOBI, bdf_schedule, butcher_tableau
 
         if opts is None:
             opts = NewtonOpts()
             opts.max_iters, opts.nl_tol, opts.backtracking = 10, 1e-12, 1
             opts.linear.solver, opts.linear.precond = MHX_KRYLOV_CG, MHX_PREC_JACOBI
             opts.linear.max_iters, opts.linear.tol = 10000, 1e-12
+        A, b, c = butcher_tableau(tableau, *(butcher or (None, None, None)))
+        one_stage = len(b) == 1 and A[0, 0] == 1.0 and b[0] == 1.0
         u = np.ascontiguousarray(u0, dtype=np.float64).copy()
         hist = [np.zeros_like(u) for _ in range(max(order, startup_order or order))]
         t = t0
         for k in bdf_schedule(order, nsteps, startup_order, startup_steps):
             hist = [u.copy()] + hist[:-1]
+            un = u.copy()
+            stages = []
+            for s in range(len(b)):
+                if at_time is not None:
+                    self.update(at_time(t + c[s] * dt))
+                if one_stage:
+                    self.set_time_state(dt, k, hist)
+                else:
+                    self.set_stage_state(dt, k, hist, A, b, s, stages)
+                us = self.setDirichlet(un.copy())
+                us, _ = self.nonlinearSolver(us, opts)
+                stages.append(us)
+            u = stages[0] if one_stage else un.copy()
+            if not one_stage:
+                for us in stages:
+                    u = u + us - un
             t = t + dt
-            if at_time is not None:
-                self.update(at_time(t))
-            self.set_time_state(dt, k, hist)
-            self.setDirichlet(u)
-            u, _ = self.nonlinearSolver(u, opts)
             if record is not None:
                 record(t, u)
         self.set_time_state(dt, 0, [])

@@ -176,11 +176,14 @@ struct Problem {
   std::vector<char> fixed;           // strong-DBC mask
   std::vector<double> dbcval;        // [ndof]
   int ad_size;
-  // transient state (Workset::computeSolnTransientSeeded, workset.cpp:601-680,
-  // BWE tableau: A = b = c = 1): u_dot_dof = alpha_t u_dof + beta_t[dof]
+  // transient state (Workset::computeSolnTransientSeeded, workset.cpp:601-680):
+  // per DOF the evaluated solution alpha_u u + beta_u and u_dot = alpha_t u +
+  // beta_t, u the stage unknown (one-stage tableaus: alpha_u = 1, no beta_u)
   int bdf = 0;
   double alpha_t = 0.0;
-  std::vector<double> beta_t;        // [ndof] sum_k BDF_k u_prev[k-1] / dt
+  std::vector<double> beta_t;        // [ndof] (sum_k BDF_k u_prev[k-1]) / (dt b_s)
+  double alpha_u = 1.0;              // A_ss / b_s
+  std::vector<double> beta_u;        // [ndof] or empty
 
   void vertex_of_elem(long e, int v, double* X) const;
   void elem_coords(long e, long* c) const {
@@ -486,15 +489,23 @@ static void assemble_thermal(const Problem& P, const double* u, bool want_jac, d
     for (long le = 0; le < nel; ++le) {
       long e = e0 + le;
       elem_geometry(P, e, G);
+      // the evaluated solution of a Butcher stage: alpha_u u + beta_u per DOF
+      const EvalT* uE = &uAD[le * nb];
+      std::vector<EvalT> ue;
+      if (!P.beta_u.empty()) {
+        ue.resize(nb);
+        for (int j = 0; j < nb; ++j) ue[j] = P.alpha_u * uAD[le * nb + j] + P.beta_u[P.lids[e * nb + j]];
+        uE = ue.data();
+      }
       // evaluateSolutionField: field = u_0 b_0, then += u_dof b_dof
       T.assign(nq, EvalT(0.0));
       dT.assign(nq * dim, EvalT(0.0));
       for (int q = 0; q < nq; ++q) {
-        T[q] = uAD[le * nb] * P.bref[q];
-        for (int a = 0; a < dim; ++a) dT[q * dim + a] = uAD[le * nb] * G.grad[(0 * nq + q) * dim + a];
+        T[q] = uE[0] * P.bref[q];
+        for (int a = 0; a < dim; ++a) dT[q * dim + a] = uE[0] * G.grad[(0 * nq + q) * dim + a];
         for (int j = 1; j < nb; ++j) {
-          T[q] += uAD[le * nb + j] * P.bref[j * nq + q];
-          for (int a = 0; a < dim; ++a) dT[q * dim + a] += uAD[le * nb + j] * G.grad[(j * nq + q) * dim + a];
+          T[q] += uE[j] * P.bref[j * nq + q];
+          for (int a = 0; a < dim; ++a) dT[q * dim + a] += uE[j] * G.grad[(j * nq + q) * dim + a];
         }
       }
       // time derivative field "T_t": the seeded nodal u_dot = alpha_t u + beta_t
@@ -1359,10 +1370,15 @@ int orc_linear_solve(void* h, const double* A, const double* b, double* x, const
 int orc_newton(void* h, double* u, const mhx_newton_opts* o, mhx_solve_stats* st) {
   return orc::newton(*(orc::Problem*)h, u, o, st);
 }
-// Backward-difference state (SolverManager::setBackwardDifference,
-// solverManager_setup.hpp:443-500: the weights exclude 1/dt, which
-// Workset::computeSolnTransientSeeded applies with b(stage) = 1 for BWE).
-int orc_set_time_state(void* h, double dt, int32_t order, const double* u_prev) {
+// Time-step / Butcher-stage state (SolverManager::setBackwardDifference,
+// solverManager_setup.hpp:443-500, weights without 1/dt; the stage terms of
+// Workset::computeSolnTransientSeeded, workset.cpp:601-680):
+//   timewt = 1/dt/b_s, alpha_t = BDF_0 timewt,
+//   beta_t = (sum_k BDF_k u_prev[k-1]) timewt,
+//   alpha_u = A_ss/b_s, beta_u = (1 - alpha_u) u_prev[0]
+//                               + sum_{j<s} A_sj/b_j (u_stages[j] - u_prev[0]).
+int orc_set_stage_state(void* h, double dt, int32_t order, const double* u_prev, int32_t nst, const double* A,
+                        const double* b, int32_t stage, const double* u_stages) {
   auto* P = (orc::Problem*)h;
   static const double W[7][7] = {
       {0}, {1.0, -1.0}, {1.5, -2.0, 0.5}, {11.0 / 6.0, -3.0, 1.5, -1.0 / 3.0},
@@ -1371,20 +1387,38 @@ int orc_set_time_state(void* h, double dt, int32_t order, const double* u_prev) 
       {147.0 / 60.0, -6.0, 15.0 / 2.0, -20.0 / 3.0, 225.0 / 60.0, -72.0 / 60.0, 1.0 / 6.0}};
   if (order < 0 || order > 6 || (order > 0 && !(dt > 0.0))) return MHX_ERR_ARG;
   P->bdf = order;
+  P->beta_u.clear();
+  P->alpha_u = 1.0;
   if (order == 0) {
     P->beta_t.clear();
     P->alpha_t = 0.0;
     return MHX_OK;
   }
-  const double timewt = 1.0 / dt;
+  if (nst < 1 || stage < 0 || stage >= nst || b[stage] == 0.0) return MHX_ERR_ARG;
+  const double timewt = 1.0 / dt / b[stage];
   P->alpha_t = W[order][0] * timewt;
   P->beta_t.assign(P->ndof, 0.0);
   for (long g = 0; g < P->ndof; ++g) {
-    double b = 0.0;
-    for (int s = 1; s <= order; ++s) b += W[order][s] * u_prev[(long)(s - 1) * P->ndof + g];
-    P->beta_t[g] = b * timewt;
+    double bt = 0.0;
+    for (int s = 1; s <= order; ++s) bt += W[order][s] * u_prev[(long)(s - 1) * P->ndof + g];
+    P->beta_t[g] = bt * timewt;
+  }
+  const double ass = A[stage * nst + stage], alpha_u = ass / b[stage];
+  if (nst > 1 || ass != 1.0 || b[stage] != 1.0) {
+    P->alpha_u = alpha_u;
+    P->beta_u.assign(P->ndof, 0.0);
+    for (long g = 0; g < P->ndof; ++g) {
+      const double u0 = u_prev[g];
+      double bu = (1.0 - alpha_u) * u0;
+      for (int j = 0; j < stage; ++j) bu += A[stage * nst + j] / b[j] * (u_stages[(long)j * P->ndof + g] - u0);
+      P->beta_u[g] = bu;
+    }
   }
   return MHX_OK;
+}
+int orc_set_time_state(void* h, double dt, int32_t order, const double* u_prev) {
+  const double one = 1.0;
+  return orc_set_stage_state(h, dt, order, u_prev, 1, &one, &one, 0, nullptr);
 }
 // coefficient update between steps: source modes, density, specific heat and
 // the constant Dirichlet value (the mesh and sides stay)

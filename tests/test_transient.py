@@ -200,3 +200,114 @@ def test_transient_bdf3_matches_oracle():
                       opts=newton_opts())
     for a, b in zip(traj_g, traj_o):
         np.testing.assert_allclose(a, b, rtol=0, atol=1e-9 * max(1.0, np.abs(b).max()))
+
+
+# --- multi-stage Butcher tableaus (DIRK, CN) ------------------------------------------
+# No reference gold uses them outside multiscale decks: the oracle is pinned by
+# exactness for polynomials in time on the ODE limit (a spatially uniform
+# solution with no boundary data: M T' = M f(t) node by node), where a stage
+# method of order p reduces to a quadrature rule, and by the measured order
+# of convergence on T' = -T, where the stages couple through A.
+TABLEAU_ORDER = {"BWE": 1, "DIRK-1,2": 2, "CN": 2, "DIRK-2,2": 2, "DIRK-2,3": 3, "DIRK-3,3": 3}
+
+
+def ode_problem(source, t=0.0):
+    return Problem(dim=2, order=1, quadrature=2, n=(2, 2), dbc_sides=0, functions={"thermal source": source},
+                   time=t)
+
+
+def ode_opts():
+    o = newton_opts()
+    o.max_iters, o.nl_tol = 8, 1e-14
+    o.linear.tol = 1e-14
+    return o
+
+
+@pytest.mark.parametrize("tableau,deg", [("BWE", 1), ("DIRK-1,2", 2), ("CN", 2), ("DIRK-2,2", 2),
+                                         ("DIRK-2,3", 4), ("DIRK-3,3", 3)])
+def test_oracle_stage_methods_exact_for_polynomials(tableau, deg):
+    from oracle import Oracle
+
+    # T = t^deg, f = deg t^(deg-1): the quadrature of each tableau is exact to
+    # its order (DIRK-2,3 is the 2-point Gauss rule: exact for t^4; DIRK-3,3's
+    # diagonal is a 10-digit constant, so its conditions hold to ~1e-10)
+    src = "1" if deg == 1 else f"{deg}*t^{deg - 1}"
+    o = Oracle(ode_problem(src))
+    dt, n = 0.1, 5
+    u = o.transientSolver(np.zeros(o.n_dof), dt, n, tableau=tableau, opts=ode_opts(),
+                          at_time=lambda t: ode_problem(src, t))
+    tol = 1e-9 if tableau == "DIRK-3,3" else 1e-12
+    np.testing.assert_allclose(u, (n * dt) ** deg, rtol=tol, atol=tol)
+
+
+@pytest.mark.parametrize("tableau", list(TABLEAU_ORDER))
+def test_oracle_stage_methods_convergence_order(tableau):
+    from oracle import Oracle
+
+    # T' = -T, T(0) = 1, errors at t = 0.8 against exp(-0.8)
+    o = Oracle(ode_problem("-T"))
+    errs = []
+    for n in (8, 16, 32):
+        u = o.transientSolver(np.ones(o.n_dof), 0.8 / n, n, tableau=tableau, opts=ode_opts())
+        errs.append(abs(u[0] - np.exp(-0.8)))
+    rates = np.log2(np.array(errs[:-1]) / np.array(errs[1:]))
+    p = TABLEAU_ORDER[tableau]
+    assert np.all(np.abs(rates - p) < 0.2), (tableau, errs, rates)
+
+
+def test_butcher_tableau_checks():
+    from mrhyde_amd.abi import butcher_tableau
+
+    for name in TABLEAU_ORDER:
+        A, b, c = butcher_tableau(name)
+        assert np.sum(b) == pytest.approx(1.0, abs=1e-12)   # consistency
+        np.testing.assert_allclose(A.sum(axis=1), c, atol=1e-9)  # row sums = c (CN included)
+    with pytest.raises(ValueError):
+        butcher_tableau("RK-4,4")
+    with pytest.raises(ValueError):
+        butcher_tableau("custom", [[0.5, 0.5], [0.0, 0.5]], [0.5, 0.5], [1.0, 0.5])  # not lower triangular
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("stage", [0, 1, 2])
+def test_stage_assembly_parity(stage):
+    """Stage `stage` of DIRK-3,3 with a BDF2 history (the general formula):
+    residual and Jacobian within 1e-12 of the oracle."""
+    from mrhyde_amd import Mhx
+    from mrhyde_amd.abi import butcher_tableau
+    from oracle import Oracle
+
+    p = TR_CONFIGS["2d_q2_nonlinear_perturbed"]
+    g, o = Mhx(p), Oracle(p)
+    A, b, _ = butcher_tableau("DIRK-3,3")
+    rng = np.random.default_rng(40 + stage)
+    hist = [rng.standard_normal(o.n_dof) for _ in range(2)]
+    stages = [rng.standard_normal(o.n_dof) for _ in range(stage)]
+    u = o.setDirichlet(rng.standard_normal(o.n_dof))
+    print(f"[transient] DIRK-3,3 stage {stage}", flush=True)
+    g.set_stage_state(0.05, 2, hist, A, b, stage, stages)
+    o.set_stage_state(0.05, 2, hist, A, b, stage, stages)
+    rg, jg = g.assembleJacRes(u)
+    ro, jo = o.assembleJacRes(u)
+    rp, _ = o.crs_pattern()
+    assert_entries_close(jg, jo, rp, what=f"stage {stage} Jacobian")
+    assert_entries_close(rg, ro, None, what=f"stage {stage} residual")
+    g.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("tableau", ["DIRK-2,3", "CN"])
+def test_stage_method_trajectory_matches_oracle(tableau):
+    from mrhyde_amd import Mhx
+    from oracle import Oracle
+
+    p = TR_CONFIGS["2d_q2_nonlinear_perturbed"]
+    g, o = Mhx(p), Oracle(p)
+    traj_g, traj_o = [], []
+    u0 = np.zeros(o.n_dof)
+    g.transientSolver(u0, 0.05, 4, tableau=tableau, record=lambda t, u: traj_g.append(u.copy()))
+    o.transientSolver(u0, 0.05, 4, tableau=tableau, record=lambda t, u: traj_o.append(u.copy()),
+                      opts=newton_opts())
+    for a, b in zip(traj_g, traj_o):
+        np.testing.assert_allclose(a, b, rtol=1e-9, atol=1e-11)
+    g.close()
