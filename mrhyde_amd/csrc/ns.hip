@@ -14,14 +14,22 @@
 // Unknowns per node: ux, uy, [uz], pr (NF = DIM+1), DOF = node * NF + field.
 // An element has NE = 2^DIM * NF unknowns; element-local index j = node*NF + f.
 //
-// Work mapping.  A group of G = next power of two >= NE lanes owns one element
-// (3-D: 32 lanes, two elements per wavefront).  Lane j carries derivative
-// slot j of the reference's element-level dual (Sacado SFad, one slot per
-// DOF, workset.cpp:884): at every cubature point it evaluates the physics in
-// a value + one-derivative dual whose seed is basis function j of its field,
-// so the derivative part of the point residual is column j of the element
-// Jacobian and the value part is the residual.  Geometry and the field values
-// at the points are computed once per element into LDS (lanes 0..NQ-1).
+// Work mapping.  A workgroup holds EPW = 64 / 2^DIM elements and one
+// wavefront per field (3-D: 8 elements, 4 wavefronts).
+//   point stage   (wavefront 0, lane = (element, point)): geometry, the fields
+//                 and their physical gradients, tau and d tau / d u, the
+//                 strong momentum residual and the point residual in
+//                 reference form, once per point, to LDS;
+//   column stage  (wavefront F, lane = (element, node n)): residual row
+//                 (n, F) from the point residuals, and Jacobian column (n, F)
+//                 as the tangent of the point residual for the seed of basis
+//                 n in field F.  The field is uniform in the wavefront, so
+//                 each wavefront runs the tangent specialised for its field;
+//                 it is the derivative part of the reference's element-level
+//                 dual (Sacado SFad, one slot per DOF, workset.cpp:884) for
+//                 that slot, written out term by term;
+//   scatter       the columns are exchanged through LDS so that a lane
+//                 writes the NF contiguous entries of a row and column node.
 // Elements of one colour share no node, so the scatter needs no atomics; as
 // in the thermal kernel the first scheduled element touching a CRS entry
 // stores it (Dirichlet rows: e_d, residual 0) and later ones add into it.
@@ -46,30 +54,20 @@ struct NsCfg {
   static constexpr int NQ = 1 << DIM;  // 2^DIM Gauss points
   static constexpr int NF = DIM + 1;   // unknowns per node
   static constexpr int NE = NV * NF;   // element unknowns
-  static constexpr int G = NE <= 16 ? 16 : 32;
-  static constexpr int GPB = 256 / G;
-  static constexpr int NQS = DIM * DIM + 1 + NF * (DIM + 1);  // per point: J^-1, w|J|, (u_f, grad u_f)
-  static_assert(NE <= G, "element unknowns exceed the lane group");
+  static constexpr int EPW = 64 / NV;  // elements per workgroup: a lane per (element, node) in each wave
+  static constexpr int NT = 64 * NF;   // one wave per field
+  static constexpr int NFK = NF * (DIM + 1);  // (field, component) pairs: value and reference gradient
+  // per point in LDS
+  static constexpr int OJ = 0;                 // J^-1
+  static constexpr int OW = DIM * DIM;         // w |J|
+  static constexpr int OF = OW + 1;            // fields: physical gradient [f*(DIM+1) + c], value [.. + DIM]
+  static constexpr int OT = OF + NFK;          // tau
+  static constexpr int OTC = OT + 1;           // d tau / d u_a
+  static constexpr int OS = OTC + DIM;         // strong momentum residual
+  static constexpr int OH = OS + DIM;          // point residual Fhat[f][k] (reference form)
+  static constexpr int NQS = OH + NFK;
+  static_assert(NV * EPW == 64, "a wave holds whole elements");
 };
-
-// value + one derivative
-struct D1 {
-  double v, d;
-};
-__device__ __forceinline__ D1 operator+(D1 a, D1 b) { return {a.v + b.v, a.d + b.d}; }
-__device__ __forceinline__ D1 operator-(D1 a, D1 b) { return {a.v - b.v, a.d - b.d}; }
-__device__ __forceinline__ D1 operator*(D1 a, D1 b) { return {a.v * b.v, a.d * b.v + a.v * b.d}; }
-__device__ __forceinline__ D1 operator*(double s, D1 a) { return {s * a.v, s * a.d}; }
-__device__ __forceinline__ D1 operator*(D1 a, double s) { return {s * a.v, s * a.d}; }
-__device__ __forceinline__ D1 operator-(D1 a, double s) { return {a.v - s, a.d}; }
-__device__ __forceinline__ D1 d1_sqrt(D1 a) {
-  const double r = sqrt(a.v);
-  return {r, a.d * (0.5 / r)};
-}
-__device__ __forceinline__ D1 d1_inv(D1 a) {
-  const double r = 1.0 / a.v;
-  return {r, -a.d * r * r};
-}
 
 // Q1 on [-1,1]: node bit b -> (1 + s xi)/2 with s = 2b - 1; Gauss points
 // +-1/sqrt(3) in descending order (point bit 0 -> +1/sqrt(3)), weights 1.
@@ -97,71 +95,185 @@ __device__ __forceinline__ void q1_basis(int n, int q, double* t) {
   t[DIM] = val;
 }
 
-// navierstokes<EvalT>::computeTau, steady (C3 = 0)
-__device__ __forceinline__ D1 ns_tau(const D1* vel, int dim, double mu, double h) {
-  const double C1 = 4.0, C2 = 2.0;
-  D1 nvel = vel[0] * vel[0];
-  for (int a = 1; a < dim; ++a) nvel = nvel + vel[a] * vel[a];
-  if (nvel.v > 1e-12) nvel = d1_sqrt(nvel);
-  const double t1 = (C1 * mu / h / h) * (C1 * mu / h / h);
-  const D1 c2 = (C2 / h) * nvel;
-  D1 t = c2 * c2;
-  t.v += t1;
-  return d1_inv(d1_sqrt(t));
+// The same values as a table in constant memory, [point][node][(d/dxi.., value)]:
+// read with a wavefront-uniform point index (scalar loads) in the column stage.
+template <int DIM>
+struct Q1Tab {
+  double t[1 << DIM][1 << DIM][DIM + 1];
+};
+template <int DIM>
+constexpr Q1Tab<DIM> make_q1tab() {
+  Q1Tab<DIM> r{};
+  for (int q = 0; q < (1 << DIM); ++q)
+    for (int n = 0; n < (1 << DIM); ++n) {
+      double val = 1.0;
+      for (int c = 0; c < DIM; ++c) {
+        double g = 1.0;
+        for (int a = 0; a < DIM; ++a) {
+          const int nb = (n >> a) & 1, qb = (q >> a) & 1;
+          g *= (a == c) ? q1_der(nb) : q1_val(nb, qb);
+        }
+        r.t[q][n][c] = g;
+        val *= q1_val((n >> c) & 1, (q >> c) & 1);
+      }
+      r.t[q][n][DIM] = val;
+    }
+  return r;
+}
+__constant__ Q1Tab<2> c_q1tab2 = make_q1tab<2>();
+__constant__ Q1Tab<3> c_q1tab3 = make_q1tab<3>();
+template <int DIM>
+__device__ __forceinline__ const Q1Tab<DIM>& q1tab();
+template <>
+__device__ __forceinline__ const Q1Tab<2>& q1tab<2>() { return c_q1tab2; }
+template <>
+__device__ __forceinline__ const Q1Tab<3>& q1tab<3>() { return c_q1tab3; }
+
+// Tangent of the point residual for a seed in field F (compile time) with
+// value phi and physical gradient gj: dFh[f][k], the derivative of the
+// reference-form point residual Fhat[f][k] (navierstokes.cpp:494-850,
+// computeTau :1053-1081), from the point values o[] (LDS).  Written as the
+// forward-mode derivative of the value expressions, term by term, with the
+// terms that vanish for field F left out at compile time.
+template <int DIM, int F>
+__device__ __forceinline__ void ns_tangent(const double* o, const NsCoef& nc, double w, double phi,
+                                           const double* gj, double dFh[DIM + 1][DIM + 1]) {
+  using C = NsCfg<DIM>;
+  constexpr bool VEL = F < DIM;
+  constexpr int NFK1 = DIM + 1;
+  const double* fld = o + C::OF;  // u_f at fld[f * NFK1 + DIM], d_c u_f at fld[f * NFK1 + c]
+  const double tau = o[C::OT];
+  // seed: du_F = phi, dgrad u_F = gj
+  double dconv[DIM], dsr[DIM];
+  double ugj = 0.0;
+  if constexpr (VEL) {
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) ugj = fma(fld[c * NFK1 + DIM], gj[c], ugj);
+  }
+#pragma unroll
+  for (int m = 0; m < DIM; ++m) {
+    // conv_m = sum_c u_c d_c u_m
+    double d = 0.0;
+    if constexpr (VEL) {
+      d = phi * fld[m * NFK1 + (VEL ? F : 0)];
+      if (m == F) d += ugj;
+    }
+    dconv[m] = d;
+    // stabres_m = rho conv_m + d_m p - rho f_m
+    dsr[m] = nc.rho * d;
+    if constexpr (!VEL) dsr[m] = gj[m];
+  }
+  const double dtau = VEL ? o[C::OTC + (VEL ? F : 0)] * phi : 0.0;
+#pragma unroll
+  for (int m = 0; m < DIM; ++m) {
+    const double sr = o[C::OS + m];
+    const double dts = dtau * sr + tau * dsr[m];  // d(tau stabres_m)
+    const double ts = tau * sr;
+    double dfl[DIM];
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) {
+      double d = 0.0;
+      if constexpr (VEL) {
+        if (m == F) d = nc.mu * gj[c];  // mu d_c u_m
+      } else {
+        if (c == m) d = -phi;  // - p delta_mc
+      }
+      if (nc.supg) {
+        // tau stabres_m u_c
+        double sd = dts * fld[c * NFK1 + DIM];
+        if constexpr (VEL) {
+          if (c == F) sd = fma(ts, phi, sd);
+        }
+        d += sd;
+      }
+      dfl[c] = d * w;
+    }
+    dFh[m][DIM] = dconv[m] * (nc.rho * w);
+#pragma unroll
+    for (int r = 0; r < DIM; ++r) {
+      double s = o[C::OJ + r * DIM] * dfl[0];
+#pragma unroll
+      for (int c = 1; c < DIM; ++c) s = fma(o[C::OJ + r * DIM + c], dfl[c], s);
+      dFh[m][r] = s;
+    }
+  }
+  // continuity: div u, PSPG tau stabres_c / rho
+  dFh[DIM][DIM] = VEL ? gj[VEL ? F : 0] * w : 0.0;
+  double dfl[DIM];
+  const double kw = w / nc.rho;
+#pragma unroll
+  for (int c = 0; c < DIM; ++c) dfl[c] = nc.pspg ? (dsr[c] * (tau * kw) + o[C::OS + c] * (dtau * kw)) : 0.0;
+#pragma unroll
+  for (int r = 0; r < DIM; ++r) {
+    double s = o[C::OJ + r * DIM] * dfl[0];
+#pragma unroll
+    for (int c = 1; c < DIM; ++c) s = fma(o[C::OJ + r * DIM + c], dfl[c], s);
+    dFh[DIM][r] = s;
+  }
 }
 
+// One workgroup: EPW elements, one wavefront per field.  Point stage (wave
+// 0, lane = (element, point)): geometry, the fields and their gradients, tau
+// and its derivative by the velocity, and the point residual in reference
+// form, once per point, to LDS.  Column stage (wave F = field F, lane =
+// (element, node n)): residual row (n, F) from the point residuals, and
+// Jacobian column (n, F) from the tangent of the point residual for the seed
+// of basis n in field F; the field is uniform in the wavefront, so each
+// wavefront runs the tangent specialised for its field.
 template <int DIM, bool JAC>
-__global__ __launch_bounds__(256) void k_ns(AsmArgs a, NsCoef nc) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_ns(AsmArgs a, NsCoef nc) {
   using C = NsCfg<DIM>;
-  constexpr int NV = C::NV, NQ = C::NQ, NF = C::NF, NE = C::NE, G = C::G, GPB = C::GPB, NQS = C::NQS;
-  __shared__ double s_u[GPB][NE];
-  __shared__ double s_X[GPB][NV * DIM];
-  __shared__ double s_q[GPB][NQ][NQS];
-  __shared__ int s_rp[GPB][NE];
+  constexpr int NV = C::NV, NQ = C::NQ, NF = C::NF, NE = C::NE, EPW = C::EPW, NQS = C::NQS;
+  __shared__ double s_u[EPW][NE];
+  __shared__ double s_X[EPW][NV * DIM];
+  __shared__ double s_q[EPW][NQ][NQS];
+  __shared__ int s_rp[EPW][NE];
 
-  const int g = threadIdx.x / G, lane = threadIdx.x % G;
-  int ec[3];
-  const bool active = element_of<DIM>(a, g, 0, ec);
-  bool first_lo[3] = {true, true, true}, first_hi[3] = {true, true, true};
-#pragma unroll
-  for (int d = 0; d < DIM; ++d) {
-    const int k = axis_key(a, d, ec[d]);
-    first_lo[d] = ec[d] == 0 || k < axis_key(a, d, ec[d] - 1);
-    first_hi[d] = ec[d] == a.m.nel[d] - 1 || k < axis_key(a, d, ec[d] + 1);
-  }
+  const int t = threadIdx.x;
+  const int wv = __builtin_amdgcn_readfirstlane(t / 64), lane = t % 64;
+  const int el = lane / NV;  // element of this lane in the workgroup (column and point stages)
 
-  // 1. gather: element unknowns (node-major, field-minor) and vertices
-  if (active) {
-    if (lane < NE) {
-      const int n = lane / NF, f = lane % NF;
+  // 1. gather: element unknowns (node-major, field-minor), CRS offsets, vertices
+  for (int k = t; k < EPW * NE; k += C::NT) {
+    const int e = k / NE, j = k % NE;
+    int ec[3];
+    const bool act = element_of<DIM>(a, e, 0, ec);
+    if (act) {
+      const int n = j / NF, f = j % NF;
       const long dof = node_of<DIM, 1>(a.m, ec, n) * NF + f;
-      s_u[g][lane] = a.u[dof];
-      s_rp[g][lane] = a.fixed[dof] ? -1 - a.rowptr[dof] : a.rowptr[dof];
+      s_u[e][j] = a.u[dof];
+      s_rp[e][j] = a.fixed[dof] ? -1 - a.rowptr[dof] : a.rowptr[dof];
     }
-    for (int t = lane; t < NV * DIM; t += G) {
-      const int v = t / DIM, d = t % DIM;
+  }
+  for (int k = t; k < EPW * NV * DIM; k += C::NT) {
+    const int e = k / (NV * DIM), r = k % (NV * DIM);
+    int ec[3];
+    if (element_of<DIM>(a, e, 0, ec)) {
+      const int v = r / DIM, d = r % DIM;
       const long vid = node_of<DIM, 1>(a.m, ec, v);  // Q1: nodes are the vertices
-      s_X[g][t] = a.vx[vid * 3 + d];
+      s_X[e][r] = a.vx[vid * 3 + d];
     }
   }
   __syncthreads();
 
-  // 2. per point (lane q): J, J^-1, w|J|, field values and physical gradients
-  if (active && lane < NQ) {
-    const int q = lane;
+  // 2. point stage, part 1: geometry and fields (wave 0, lane = (element, point))
+  int ecp[3];
+  const bool pact = wv == 0 && element_of<DIM>(a, el, 0, ecp);
+  const int q = lane % NQ;
+  if (pact) {
     double J[DIM][DIM];
 #pragma unroll
     for (int r = 0; r < DIM; ++r)
 #pragma unroll
-      for (int s = 0; s < DIM; ++s) J[r][s] = 0.0;
+      for (int s2 = 0; s2 < DIM; ++s2) J[r][s2] = 0.0;
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
-      double t[DIM + 1];
-      q1_basis<DIM>(v, q, t);
+      double tv[DIM + 1];
+      q1_basis<DIM>(v, q, tv);
 #pragma unroll
       for (int r = 0; r < DIM; ++r)
 #pragma unroll
-        for (int s = 0; s < DIM; ++s) J[r][s] = fma(s_X[g][v * DIM + r], t[s], J[r][s]);
+        for (int s2 = 0; s2 < DIM; ++s2) J[r][s2] = fma(s_X[el][v * DIM + r], tv[s2], J[r][s2]);
     }
     double Ji[DIM][DIM], det;
     if constexpr (DIM == 2) {
@@ -185,13 +297,12 @@ __global__ __launch_bounds__(256) void k_ns(AsmArgs a, NsCoef nc) {
       Ji[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id;
       Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
     }
-    double* o = s_q[g][q];
+    double* o = s_q[el][q];
 #pragma unroll
     for (int r = 0; r < DIM; ++r)
 #pragma unroll
-      for (int s = 0; s < DIM; ++s) o[r * DIM + s] = Ji[r][s];
-    o[DIM * DIM] = fabs(det);  // Gauss weights are 1
-    // fields: value and reference gradient, then physical gradient J^-T grad^
+      for (int s2 = 0; s2 < DIM; ++s2) o[C::OJ + r * DIM + s2] = Ji[r][s2];
+    o[C::OW] = fabs(det);  // Gauss weights are 1
     double fv[NF][DIM + 1];
 #pragma unroll
     for (int f = 0; f < NF; ++f)
@@ -199,138 +310,177 @@ __global__ __launch_bounds__(256) void k_ns(AsmArgs a, NsCoef nc) {
       for (int c = 0; c <= DIM; ++c) fv[f][c] = 0.0;
 #pragma unroll
     for (int n = 0; n < NV; ++n) {
-      double t[DIM + 1];
-      q1_basis<DIM>(n, q, t);
+      double tv[DIM + 1];
+      q1_basis<DIM>(n, q, tv);
 #pragma unroll
       for (int f = 0; f < NF; ++f) {
-        const double un = s_u[g][n * NF + f];
+        const double un = s_u[el][n * NF + f];
 #pragma unroll
-        for (int c = 0; c <= DIM; ++c) fv[f][c] = fma(un, t[c], fv[f][c]);
+        for (int c = 0; c <= DIM; ++c) fv[f][c] = fma(un, tv[c], fv[f][c]);
       }
     }
 #pragma unroll
     for (int f = 0; f < NF; ++f) {
-      double* of = o + DIM * DIM + 1 + f * (DIM + 1);
+      double* of = o + C::OF + f * (DIM + 1);
       of[DIM] = fv[f][DIM];
 #pragma unroll
       for (int c = 0; c < DIM; ++c) {
-        double s = 0.0;
+        double s2 = 0.0;
 #pragma unroll
-        for (int b = 0; b < DIM; ++b) s = fma(Ji[b][c], fv[f][b], s);
-        of[c] = s;
+        for (int b = 0; b < DIM; ++b) s2 = fma(Ji[b][c], fv[f][b], s2);
+        of[c] = s2;
       }
     }
   }
   __syncthreads();
 
-  // 3. lane j: residual row j and Jacobian column j
-  const bool col = active && lane < NE;
-  const int jn = lane / NF, jb = lane % NF;
+  // 2'. point stage, part 2: tau (Workset::getElementSize h), convection,
+  // strong momentum residual and the point residual values
+  if (pact) {
+    double* o = s_q[el][q];
+    double vol = 0.0;
+#pragma unroll
+    for (int qq = 0; qq < NQ; ++qq) vol += s_q[el][qq][C::OW];
+    const double h = DIM == 2 ? sqrt(vol) : cbrt(vol);
+    const double w = o[C::OW];
+    const double* fld = o + C::OF;
+    double U[NF], GU[NF][DIM];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      U[f] = fld[f * (DIM + 1) + DIM];
+#pragma unroll
+      for (int c = 0; c < DIM; ++c) GU[f][c] = fld[f * (DIM + 1) + c];
+    }
+    // computeTau, steady (C3 = 0): tau = (t1 + (C2/h |u|)^2)^(-1/2), with
+    // |u|^2 used as |u| below 1e-12; its derivative by u_a
+    double tau = 0.0, tc[DIM];
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) tc[c] = 0.0;
+    if (nc.supg || nc.pspg) {
+      const double C1 = 4.0, C2 = 2.0;
+      double nv = U[0] * U[0];
+#pragma unroll
+      for (int c = 1; c < DIM; ++c) nv = nv + U[c] * U[c];
+      double dn = 2.0;  // d nvel / d(|u|^2) times 2 u_a
+      if (nv > 1e-12) {
+        const double r = sqrt(nv);
+        dn = 2.0 * (0.5 / r);
+        nv = r;
+      }
+      const double t1 = (C1 * nc.mu / h / h) * (C1 * nc.mu / h / h);
+      const double c2 = (C2 / h) * nv;
+      const double tt = c2 * c2 + t1;
+      const double sq = sqrt(tt);
+      tau = 1.0 / sq;
+      // d tau = -(d tt (0.5/sq)) tau^2,  d tt = 2 c2 (C2/h) d nvel
+      const double k = -(0.5 / sq) * tau * tau * (2.0 * c2) * (C2 / h) * dn;
+#pragma unroll
+      for (int c = 0; c < DIM; ++c) tc[c] = k * U[c];
+    }
+    o[C::OT] = tau;
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) o[C::OTC + c] = tc[c];
+    double conv[DIM], sr[DIM];
+#pragma unroll
+    for (int m = 0; m < DIM; ++m) {
+      double cv = U[0] * GU[m][0];
+#pragma unroll
+      for (int c = 1; c < DIM; ++c) cv = cv + U[c] * GU[m][c];
+      conv[m] = cv;
+      sr[m] = (nc.rho * cv + GU[DIM][m]) - nc.rho * nc.force[m];
+      o[C::OS + m] = sr[m];
+    }
+    const double pr = U[DIM];
+    double Fh[NF][DIM + 1];
+#pragma unroll
+    for (int m = 0; m < DIM; ++m) {
+      double fl[DIM];
+#pragma unroll
+      for (int c = 0; c < DIM; ++c) {
+        double v = nc.mu * GU[m][c];
+        if (c == m) v = v - pr;
+        if (nc.supg) v = v + tau * sr[m] * U[c];
+        fl[c] = v * w;
+      }
+      Fh[m][DIM] = (conv[m] - nc.force[m]) * (nc.rho * w);
+#pragma unroll
+      for (int r = 0; r < DIM; ++r) {
+        double s2 = o[C::OJ + r * DIM] * fl[0];
+#pragma unroll
+        for (int c = 1; c < DIM; ++c) s2 = s2 + o[C::OJ + r * DIM + c] * fl[c];
+        Fh[m][r] = s2;
+      }
+    }
+    {
+      double divu = GU[0][0];
+#pragma unroll
+      for (int c = 1; c < DIM; ++c) divu = divu + GU[c][c];
+      Fh[DIM][DIM] = divu * w;
+      double fl[DIM];
+#pragma unroll
+      for (int c = 0; c < DIM; ++c) fl[c] = nc.pspg ? sr[c] * (tau * (w / nc.rho)) : 0.0;
+#pragma unroll
+      for (int r = 0; r < DIM; ++r) {
+        double s2 = o[C::OJ + r * DIM] * fl[0];
+#pragma unroll
+        for (int c = 1; c < DIM; ++c) s2 = s2 + o[C::OJ + r * DIM + c] * fl[c];
+        Fh[DIM][r] = s2;
+      }
+    }
+#pragma unroll
+    for (int f = 0; f < NF; ++f)
+#pragma unroll
+      for (int k = 0; k <= DIM; ++k) o[C::OH + f * (DIM + 1) + k] = Fh[f][k];
+  }
+  __syncthreads();
+
+  // 3. column stage: wave F, lane (element el, node jn): residual row and
+  // Jacobian column (jn, F)
+  int ec[3];
+  const bool active = wv < NF && element_of<DIM>(a, el, 0, ec);
+  const int F = wv, jn = lane % NV, jl = jn * NF + F;
   double R[JAC ? NE : 1];
 #pragma unroll
   for (int i = 0; i < (JAC ? NE : 1); ++i) R[i] = 0.0;
   double rres = 0.0;
-  if (col) {
-    double vol = 0.0;
-#pragma unroll
-    for (int q = 0; q < NQ; ++q) vol += s_q[g][q][DIM * DIM];
-    const double h = DIM == 2 ? sqrt(vol) : cbrt(vol);  // Workset::getElementSize
-    const bool stab = nc.supg || nc.pspg;
-    // a runtime point loop: unrolled, every point's temporaries stay live
+  if (active) {
+    const Q1Tab<DIM>& tab = q1tab<DIM>();
+    // a runtime point loop (uniform index): unrolled, the points' LDS reads
+    // are hoisted together and the registers run out
 #pragma unroll 1
-    for (int q = 0; q < NQ; ++q) {
-      const double* o = s_q[g][q];
-      double Ji[DIM][DIM];
+    for (int qq = 0; qq < NQ; ++qq) {
+      const double* o = s_q[el][qq];
+      // the lane's basis at the point (reference gradient, value)
+      double tjn[DIM + 1];
 #pragma unroll
-      for (int r = 0; r < DIM; ++r)
+      for (int k = 0; k <= DIM; ++k) tjn[k] = tab.t[qq][jn][k];
+      // residual row (jn, F)
 #pragma unroll
-        for (int s = 0; s < DIM; ++s) Ji[r][s] = o[r * DIM + s];
-      const double w = o[DIM * DIM];
-      // seed: basis function jn of field jb (reference and physical gradient)
-      double tj[DIM + 1];
-      q1_basis<DIM>(jn, q, tj);
-      double gj[DIM];
-#pragma unroll
-      for (int c = 0; c < DIM; ++c) {
-        double s = 0.0;
-#pragma unroll
-        for (int b = 0; b < DIM; ++b) s = fma(Ji[b][c], tj[b], s);
-        gj[c] = s;
-      }
-      D1 U[NF], GU[NF][DIM];
-#pragma unroll
-      for (int f = 0; f < NF; ++f) {
-        const double* of = o + DIM * DIM + 1 + f * (DIM + 1);
-        const bool seeded = JAC && f == jb;
-        U[f] = {of[DIM], seeded ? tj[DIM] : 0.0};
-#pragma unroll
-        for (int c = 0; c < DIM; ++c) GU[f][c] = {of[c], seeded ? gj[c] : 0.0};
-      }
-      const D1 pr = U[DIM];
-      const D1 tau = stab ? ns_tau(U, DIM, nc.mu, h) : D1{0.0, 0.0};
-      D1 conv[DIM], stabres[DIM];
-#pragma unroll
-      for (int m = 0; m < DIM; ++m) {
-        conv[m] = U[0] * GU[m][0];
-#pragma unroll
-        for (int c = 1; c < DIM; ++c) conv[m] = conv[m] + U[c] * GU[m][c];
-        stabres[m] = (nc.rho * conv[m] + GU[DIM][m]) - nc.rho * nc.force[m];
-      }
-      // point residual in reference form: Fhat_f = (J^-1 flux_f, src_f)
-      D1 Fh[NF][DIM + 1];
-#pragma unroll
-      for (int m = 0; m < DIM; ++m) {
-        D1 fl[DIM];
+      for (int k = 0; k <= DIM; ++k) rres = fma(o[C::OH + F * (DIM + 1) + k], tjn[k], rres);
+      if constexpr (JAC) {
+        double gj[DIM];  // physical gradient J^-T grad^
 #pragma unroll
         for (int c = 0; c < DIM; ++c) {
-          fl[c] = nc.mu * GU[m][c];
-          if (c == m) fl[c] = fl[c] - pr;
-          if (nc.supg) fl[c] = fl[c] + tau * stabres[m] * U[c];
-          fl[c] = fl[c] * w;
+          double s2 = 0.0;
+#pragma unroll
+          for (int b = 0; b < DIM; ++b) s2 = fma(o[C::OJ + b * DIM + c], tjn[b], s2);
+          gj[c] = s2;
         }
-        Fh[m][DIM] = (conv[m] - nc.force[m]) * (nc.rho * w);
-#pragma unroll
-        for (int r = 0; r < DIM; ++r) {
-          D1 s = Ji[r][0] * fl[0];
-#pragma unroll
-          for (int c = 1; c < DIM; ++c) s = s + Ji[r][c] * fl[c];
-          Fh[m][r] = s;
-        }
-      }
-      {
-        D1 divu = GU[0][0];
-#pragma unroll
-        for (int c = 1; c < DIM; ++c) divu = divu + GU[c][c];
-        Fh[DIM][DIM] = divu * w;
-        D1 fl[DIM];
-#pragma unroll
-        for (int c = 0; c < DIM; ++c) fl[c] = nc.pspg ? stabres[c] * (tau * (w / nc.rho)) : D1{0.0, 0.0};
-#pragma unroll
-        for (int r = 0; r < DIM; ++r) {
-          D1 s = Ji[r][0] * fl[0];
-#pragma unroll
-          for (int c = 1; c < DIM; ++c) s = s + Ji[r][c] * fl[c];
-          Fh[DIM][r] = s;
-        }
-      }
-      // residual row (jn, jb): value part against test function jn
-#pragma unroll
-      for (int f = 0; f < NF; ++f)
-        if (f == jb)
-#pragma unroll
-          for (int k = 0; k <= DIM; ++k) rres = fma(Fh[f][k].v, tj[k], rres);
-      // Jacobian column j: derivative part against every test function
-      if constexpr (JAC) {
+        double dFh[DIM + 1][DIM + 1];
+        const double w = o[C::OW];
+        if (F == 0) ns_tangent<DIM, 0>(o, nc, w, tjn[DIM], gj, dFh);
+        else if (F == 1) ns_tangent<DIM, 1>(o, nc, w, tjn[DIM], gj, dFh);
+        else if (DIM == 3 && F == 2) ns_tangent<DIM, (DIM == 3 ? 2 : 0)>(o, nc, w, tjn[DIM], gj, dFh);
+        else ns_tangent<DIM, DIM>(o, nc, w, tjn[DIM], gj, dFh);
+        // column: against every test function (uniform table reads)
 #pragma unroll
         for (int in = 0; in < NV; ++in) {
-          double ti[DIM + 1];
-          q1_basis<DIM>(in, q, ti);
 #pragma unroll
           for (int f = 0; f < NF; ++f) {
-            double s = R[in * NF + f];
+            double s2 = R[in * NF + f];
 #pragma unroll
-            for (int k = 0; k <= DIM; ++k) s = fma(Fh[f][k].d, ti[k], s);
-            R[in * NF + f] = s;
+            for (int k = 0; k <= DIM; ++k) s2 = fma(dFh[f][k], tab.t[qq][in][k], s2);
+            R[in * NF + f] = s2;
           }
         }
       }
@@ -338,62 +488,84 @@ __global__ __launch_bounds__(256) void k_ns(AsmArgs a, NsCoef nc) {
   }
 
   // 4. scatter: first scheduled element touching an entry stores it
+  bool first_lo[3] = {true, true, true}, first_hi[3] = {true, true, true};
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) {
+    const int k = axis_key(a, d, ec[d]);
+    first_lo[d] = ec[d] == 0 || k < axis_key(a, d, ec[d] - 1);
+    first_hi[d] = ec[d] == a.m.nel[d] - 1 || k < axis_key(a, d, ec[d] + 1);
+  }
   auto shared_first = [&](int ni, int nj) {
     bool f = true;
 #pragma unroll
     for (int d = 0; d < DIM; ++d) {
-      const int il = (ni >> d) & 1, jl = (nj >> d) & 1;
-      if (il == jl) f = f && (il == 0 ? first_lo[d] : first_hi[d]);
+      const int il = (ni >> d) & 1, jl2 = (nj >> d) & 1;
+      if (il == jl2) f = f && (il == 0 ? first_lo[d] : first_hi[d]);
     }
     return f;
   };
-  if (col && a.res) {
-    const long row = node_of<DIM, 1>(a.m, ec, jn) * NF + jb;
-    const bool fx = s_rp[g][lane] < 0;
+  if (active && a.res) {
+    const long row = node_of<DIM, 1>(a.m, ec, jn) * NF + F;
+    const bool fx = s_rp[el][jl] < 0;
     if (a.first_touch && shared_first(jn, jn)) a.res[row] = fx ? 0.0 : -rres;
     else if (!fx) a.res[row] += -rres;
   }
   if constexpr (JAC) {
-    if (col && a.jac) {
-      const int jl[3] = {jn & 1, (jn >> 1) & 1, (jn >> 2) & 1};
-      constexpr int CH = 8;
+    // Column (jn, F) of every element sits in the lanes of wave F.  It is
+    // exchanged through LDS (over the point data, no longer read), the rows
+    // of two nodes per round, so that a lane then reads back and writes the
+    // NF contiguous entries of one row and one column node, back to back.
+    // Plain stores: the L2 merges a lane's NF stores into whole lines
+    // (non-temporal stores of the same pattern went out as partial writes,
+    // 2-4x the bytes).
+    constexpr int RC = 2 * NF;  // rows per round
+    static_assert(RC * EPW * NV * NF <= EPW * NQ * NQS, "the exchange fits over the point data");
+    double* xb = &s_q[0][0][0];
+    const int jlc[3] = {jn & 1, (jn >> 1) & 1, (jn >> 2) & 1};
+    __syncthreads();  // every wavefront is done with the point data
 #pragma unroll
-      for (int i0 = 0; i0 < NE; i0 += CH) {
-        int sl[CH];
-        uint32_t first_mask = 0, fixed_mask = 0;
-        double old[CH];
+    for (int i0 = 0; i0 < NE; i0 += RC) {
+      if (active) {
 #pragma unroll
-        for (int k = 0; k < CH; ++k) {
-          const int i = i0 + k;
-          if (i >= NE) break;
-          const int in = i / NF;
+        for (int r = 0; r < RC; ++r) xb[((r * EPW + el) * NV + jn) * NF + F] = R[i0 + r];
+      }
+      __syncthreads();
+      if (active && a.jac) {
+        // rows (in, F) for the round's two nodes in; lane: column node jn
+        long base[2];
+        bool first[2], fx[2];
+        double old[2][NF];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int in = i0 / NF + h, i = in * NF + F;
           const int il[3] = {in & 1, (in >> 1) & 1, (in >> 2) & 1};
           int px, lx, py, ly, pz = 0, lz = 1;
-          axis_slot<1>(il[0], jl[0], ec[0], a.m.nn[0], px, lx);
-          axis_slot<1>(il[1], jl[1], ec[1], a.m.nn[1], py, ly);
-          if (DIM == 3) axis_slot<1>(il[2], jl[2], ec[2], a.m.nn[2], pz, lz);
-          int rp = s_rp[g][i];
-          const bool fx = rp < 0;
-          if (fx) rp = -1 - rp;
-          sl[k] = rp + ((pz * ly + py) * lx + px) * NF + jb;
-          const bool first = a.first_touch && shared_first(in, jn);
-          if (first) first_mask |= 1u << k;
-          if (fx) fixed_mask |= 1u << k;
-          old[k] = (!first && !fx) ? a.jac[sl[k]] : 0.0;
+          axis_slot<1>(il[0], jlc[0], ec[0], a.m.nn[0], px, lx);
+          axis_slot<1>(il[1], jlc[1], ec[1], a.m.nn[1], py, ly);
+          if (DIM == 3) axis_slot<1>(il[2], jlc[2], ec[2], a.m.nn[2], pz, lz);
+          int rp = s_rp[el][i];
+          fx[h] = rp < 0;
+          if (fx[h]) rp = -1 - rp;
+          base[h] = rp + (long)((pz * ly + py) * lx + px) * NF;
+          first[h] = a.first_touch && shared_first(in, jn);
+#pragma unroll
+          for (int f = 0; f < NF; ++f) old[h][f] = (!first[h] && !fx[h]) ? a.jac[base[h] + f] : 0.0;
         }
 #pragma unroll
-        for (int k = 0; k < CH; ++k) {
-          const int i = i0 + k;
-          if (i >= NE) break;
-          const bool first = (first_mask >> k) & 1u, fx = (fixed_mask >> k) & 1u;
-          // streaming stores (a later colour re-reads the entry long after this launch)
-          if (fx) {
-            if (first) __builtin_nontemporal_store((i == lane) ? 1.0 : 0.0, &a.jac[sl[k]]);
-          } else {
-            __builtin_nontemporal_store(old[k] + R[i], &a.jac[sl[k]]);
+        for (int h = 0; h < 2; ++h) {
+          const int in = i0 / NF + h, r = h * NF + F;
+          const double* x = xb + ((r * EPW + el) * NV + jn) * NF;
+#pragma unroll
+          for (int f = 0; f < NF; ++f) {
+            if (fx[h]) {
+              if (first[h]) a.jac[base[h] + f] = (in == jn && f == F) ? 1.0 : 0.0;
+            } else {
+              a.jac[base[h] + f] = old[h][f] + x[f];
+            }
           }
         }
       }
+      __syncthreads();
     }
   }
 }
@@ -402,7 +574,7 @@ template <int DIM>
 static void launch_ns(Ctx& c, AsmArgs& a, const NsCoef& nc, bool jac) {
   using C = NsCfg<DIM>;
   const int ncol = 1 << DIM;
-  a.gpb = C::GPB;
+  a.gpb = C::EPW;
   a.blocked = 0;
   a.nsub = 1;
   for (int col = 0; col < ncol; ++col) {
@@ -417,12 +589,12 @@ static void launch_ns(Ctx& c, AsmArgs& a, const NsCoef& nc, bool jac) {
     }
     a.ncolor = cnt;
     if (cnt == 0) continue;
-    const long blocks = (cnt + C::GPB - 1) / C::GPB;
+    const long blocks = (cnt + C::EPW - 1) / C::EPW;
     ScopedPhase ph(c.prof, jac ? "k_ns<jacres>" : "k_ns<res>", c.stream);
     if (jac)
-      hipLaunchKernelGGL((k_ns<DIM, true>), dim3((unsigned)blocks), dim3(256), 0, c.stream, a, nc);
+      hipLaunchKernelGGL((k_ns<DIM, true>), dim3((unsigned)blocks), dim3(C::NT), 0, c.stream, a, nc);
     else
-      hipLaunchKernelGGL((k_ns<DIM, false>), dim3((unsigned)blocks), dim3(256), 0, c.stream, a, nc);
+      hipLaunchKernelGGL((k_ns<DIM, false>), dim3((unsigned)blocks), dim3(C::NT), 0, c.stream, a, nc);
     MHX_HIP(hipGetLastError());
   }
 }
