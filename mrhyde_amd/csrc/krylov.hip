@@ -440,12 +440,12 @@ static void prec_apply(Ctx& c, const Prec<T>& P, const T* v, T* out) {
   MHX_HIP(hipGetLastError());
 }
 
-// w = A M^-1 v (the point-Jacobi scaling is fused into the SpMV)
+// w = A M^-1 v.  Point Jacobi is its own pass (3 vector streams) ahead of the
+// plain product: scaling the gathered x inside the product gathers D^-1 too,
+// which cost more than the pass (1.38 against 1.13 + 0.04 ms at 7.2M rows).
 template <class T>
 static void op_apply(Ctx& c, const T* A, const Prec<T>& P, const T* v, T* w, T* tmp) {
-  if (P.kind == MHX_PREC_JACOBI) {
-    spmv_t<T, T>(c, A, P.d, v, w);
-  } else if (P.kind == MHX_PREC_BLOCK_JACOBI || P.kind == MHX_PREC_AMG) {
+  if (P.kind == MHX_PREC_JACOBI || P.kind == MHX_PREC_BLOCK_JACOBI || P.kind == MHX_PREC_AMG) {
     prec_apply<T>(c, P, v, tmp);
     spmv_t<T, T>(c, A, nullptr, tmp, w);
   } else {
@@ -584,33 +584,89 @@ static int cg_core(Ctx& c, const T* A, const Prec<T>& P, const T* b, T* x, doubl
 }
 
 // ---- GMRES(m), right preconditioning, CGS + DGKS re-orthogonalisation --------
+// The Gram-Schmidt kernels deal the K basis vectors to the four wavefronts of
+// a block in contiguous chunks of ceil(K/4): a lane walks rows i, its
+// wavefront's vectors in registers, and the four wavefronts share the lines
+// of w, which HBM then delivers once per sweep.
+constexpr int kGsChunk = 17;  // 4 x 17 >= 65 basis vectors
+
 // h[k] = V_k . w for k < K  (V row-major: vector k at V + k*ld), FP64 sums
 template <class T>
 __global__ __launch_bounds__(256) void k_mdot(long n, int K, const T* __restrict__ V, long ld,
                                               const T* __restrict__ w, double* __restrict__ part,
                                               const double* __restrict__ flag) {
   if (flag && *flag == 0.0) return;
-  // K <= 65 handled in chunks of 8 to bound registers
-  __shared__ double sh[8][4];
-  for (int k0 = 0; k0 < K; k0 += 8) {
-    double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    const int kk = min(8, K - k0);
-    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
-      const double wi = (double)w[i];
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / 64), l = threadIdx.x % 64;
+  const int ch = (K + 3) / 4, k0 = wv * ch, kc = max(0, min(ch, K - k0));
+  double acc[kGsChunk];
 #pragma unroll
-      for (int k = 0; k < 8; ++k)
-        if (k < kk) v[k] = fma((double)V[(long)(k0 + k) * ld + i], wi, v[k]);
-    }
-    const int wv = threadIdx.x / 64, l = threadIdx.x % 64;
+  for (int j = 0; j < kGsChunk; ++j) acc[j] = 0.0;
+  for (long i = (long)blockIdx.x * 64 + l; i < n; i += (long)gridDim.x * 64) {
+    const double wi = (double)w[i];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      double s = wave_sum(v[k]);
-      if (l == 0) sh[k][wv] = s;
+    for (int j = 0; j < kGsChunk; ++j)
+      if (j < kc) acc[j] = fma((double)V[(long)(k0 + j) * ld + i], wi, acc[j]);
+  }
+#pragma unroll
+  for (int j = 0; j < kGsChunk; ++j) {
+    if (j < kc) {
+      const double sj = wave_sum(acc[j]);
+      if (l == 0) part[(long)(k0 + j) * kRedBlocks + blockIdx.x] = sj;
     }
-    __syncthreads();
-    if (threadIdx.x < kk) part[(long)(k0 + threadIdx.x) * kRedBlocks + blockIdx.x] =
-        sh[threadIdx.x][0] + sh[threadIdx.x][1] + sh[threadIdx.x][2] + sh[threadIdx.x][3];
-    __syncthreads();
+  }
+}
+// One sweep for the first pass's update and the second pass's coefficients:
+// w' = w - sum_k h[k] V_k (FP64), partial V_k . w' in slots 0..K-1 and
+// ||w'||^2 in slot K.  A lane keeps its chunk's V entries of row i in
+// registers from the update to the dot products, so the basis is read once
+// for both (the four chunk sums of the update meet in LDS).
+template <class T>
+__global__ __launch_bounds__(256) void k_mupdate_dot(long n, int K, const T* __restrict__ V, long ld,
+                                                     const double* __restrict__ h, T* __restrict__ w,
+                                                     double* __restrict__ part) {
+  __shared__ double s_p[2][4][64];
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / 64), l = threadIdx.x % 64;
+  const int ch = (K + 3) / 4, k0 = wv * ch, kc = max(0, min(ch, K - k0));
+  double hk[kGsChunk], acc[kGsChunk];
+#pragma unroll
+  for (int j = 0; j < kGsChunk; ++j) {
+    hk[j] = j < kc ? h[k0 + j] : 0.0;
+    acc[j] = 0.0;
+  }
+  double nrm = 0.0;
+  int buf = 0;
+  // every thread of the block runs the same trip count (the barrier below)
+  for (long base = (long)blockIdx.x * 64; base < n; base += (long)gridDim.x * 64, buf ^= 1) {
+    const long i = base + l;
+    const bool in = i < n;
+    double v[kGsChunk];
+    double p = 0.0;
+#pragma unroll
+    for (int j = 0; j < kGsChunk; ++j) {
+      v[j] = (j < kc && in) ? (double)V[(long)(k0 + j) * ld + i] : 0.0;
+      p = fma(-hk[j], v[j], p);
+    }
+    s_p[buf][wv][l] = p;
+    __syncthreads();  // (two buffers: the next sweep step writes the other one)
+    const double wi = in ? (double)w[i] : 0.0;
+    const T ws = (T)((wi + (s_p[buf][0][l] + s_p[buf][1][l])) + (s_p[buf][2][l] + s_p[buf][3][l]));
+    if (in && wv == 0) {
+      w[i] = ws;
+      nrm = fma((double)ws, (double)ws, nrm);
+    }
+#pragma unroll
+    for (int j = 0; j < kGsChunk; ++j) acc[j] = fma(v[j], (double)ws, acc[j]);
+  }
+#pragma unroll
+  for (int j = 0; j < kGsChunk; ++j) {
+    if (j < kc) {
+      const double sj = wave_sum(acc[j]);
+      if (l == 0) part[(long)(k0 + j) * kRedBlocks + blockIdx.x] = sj;
+    }
+  }
+  if (wv == 0) {
+    const double sn = wave_sum(nrm);
+    if (l == 0) part[(long)K * kRedBlocks + blockIdx.x] = sn;
   }
 }
 // w -= sum_k h[k] V_k (FP64 combination) ; partial ||w||^2
@@ -673,7 +729,7 @@ static int gmres_core(Ctx& c, const T* A, const Prec<T>& P, const T* b, T* x, in
   m = std::max(1, std::min(m, 64));
   v.r.alloc(n); v.q.alloc(n); v.tmp.alloc(n); v.tmp2.alloc(n);
   if (v.V.n < (size_t)(m + 1) * n) v.V.alloc((size_t)(m + 1) * n);
-  ensure_red(c, m + 1);
+  ensure_red(c, m + 2);
   const int nb = red_blocks(n);
   const double bn = norm2_t<T>(c, b, n);
   if (bn == 0.0) {
@@ -709,18 +765,23 @@ static int gmres_core(Ctx& c, const T* A, const Prec<T>& P, const T* b, T* x, in
       op_apply<T>(c, A, P, vk, vk1, v.tmp.p);  // w = A M^-1 v_k, stored in V_{k+1}
       // classical Gram-Schmidt with one DGKS re-orthogonalisation pass, all
       // decisions on the device; the host reads the coefficients of both
-      // passes, the flag and ||w|| with one copy per iteration
+      // passes, the flag and ||w|| with one copy per iteration.  The first
+      // pass's update also forms the second pass's coefficients (one read of
+      // the basis for both); the second pass's update runs when DGKS asks.
       double* hs = dscal + kScalW;
-      for (int pass = 0; pass < 2; ++pass) {
-        const double* fl = pass == 0 ? nullptr : hs + 2;
-        double* hp = dscal + (pass == 0 ? kScalH : kScalH2);
-        hipLaunchKernelGGL(k_mdot<T>, dim3(nb), dim3(256), 0, c.stream, n, k + 1, v.V.p, n, vk1, c.kw.part.p, fl);
-        hipLaunchKernelGGL(k_final, dim3(k + 1), dim3(256), 0, c.stream, c.kw.part.p, nb, 0, hp, fl);
-        hipLaunchKernelGGL(k_mupdate<T>, dim3(nb), dim3(256), 0, c.stream, n, k + 1, v.V.p, n, hp, vk1, c.kw.part.p,
-                           fl);
-        hipLaunchKernelGGL(k_final, dim3(1), dim3(256), 0, c.stream, c.kw.part.p, nb, 0, hs + pass, fl);
-        if (pass == 0) hipLaunchKernelGGL(k_dgks, dim3(1), dim3(64), 0, c.stream, k + 1, dscal + kScalH, hs);
-      }
+      const double* fl = hs + 2;
+      double* part = c.kw.part.p;
+      hipLaunchKernelGGL(k_mdot<T>, dim3(nb), dim3(256), 0, c.stream, n, k + 1, v.V.p, n, vk1, part, nullptr);
+      hipLaunchKernelGGL(k_final, dim3(k + 1), dim3(256), 0, c.stream, part, nb, 0, dscal + kScalH, nullptr);
+      hipLaunchKernelGGL(k_mupdate_dot<T>, dim3(nb), dim3(256), 0, c.stream, n, k + 1, v.V.p, n, dscal + kScalH,
+                         vk1, part);
+      hipLaunchKernelGGL(k_final, dim3(k + 1), dim3(256), 0, c.stream, part, nb, 0, dscal + kScalH2, nullptr);
+      hipLaunchKernelGGL(k_final, dim3(1), dim3(256), 0, c.stream, part + (long)(k + 1) * kRedBlocks, nb, 0, hs,
+                         nullptr);
+      hipLaunchKernelGGL(k_dgks, dim3(1), dim3(64), 0, c.stream, k + 1, dscal + kScalH, hs);
+      hipLaunchKernelGGL(k_mupdate<T>, dim3(nb), dim3(256), 0, c.stream, n, k + 1, v.V.p, n, dscal + kScalH2, vk1,
+                         part, fl);
+      hipLaunchKernelGGL(k_final, dim3(1), dim3(256), 0, c.stream, part, nb, 0, hs + 1, fl);
       hipLaunchKernelGGL(k_normalize<T>, grid1(n), dim3(256), 0, c.stream, n, hs, vk1);
       MHX_HIP(hipGetLastError());
       MHX_HIP(hipMemcpyAsync(hh.data(), dscal, (kScalW + 4) * sizeof(double), hipMemcpyDeviceToHost, c.stream));
