@@ -191,6 +191,13 @@ typedef struct mhx_newton_opts {
   int32_t backtracking;    /* "allow backtracking" (default true) */
   int32_t flags;           /* MHX_DEVICE_PTRS */
   mhx_krylov_opts linear;
+  /* Convergence tests (solverManager_solvers.hpp:430-445, options of
+   * solverManager_construct.hpp:59-63): the relative test (scaled inf-norm
+   * < nl_tol, or inf-norm < 1e-100) unless relative_off; with relative_off
+   * and use_abs_tol, inf-norm < abs_tol; with neither, max_iters steps. */
+  int32_t relative_off;    /* "use relative TOL" = false (default: the relative test) */
+  int32_t use_abs_tol;     /* "use absolute TOL" (default false) */
+  double  abs_tol;         /* "absolute nonlinear TOL" (0 -> min(nl_tol, 1e-6)) */
 } mhx_newton_opts;
 
 typedef struct mhx_solve_stats {
@@ -515,7 +522,9 @@ int  mhx_ctx_update(mhx_ctx* ctx, const mhx_problem_desc* desc);
  * requests run at once, each on its own host worker and HIP stream; a queued
  * request is admitted (first fit) when a worker is free and its footprint
  * (mhx_footprint) fits hbm_budget next to the contexts held; an idle context
- * with the same mesh is reused (max_cached idle contexts are kept).  Results
+ * with the same mesh is reused (max_cached idle contexts are kept; while
+ * requests are queued the cache may hold more, within hbm_budget, since a
+ * context's destruction synchronises the device).  Results
  * do not depend on the mix: reductions run in a fixed order per request.
  * No counterpart in the reference (its UQ loop runs samples one by one,
  * src/managers/analysis/analysisManager_solve.hpp). */

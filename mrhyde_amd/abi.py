@@ -78,7 +78,7 @@ class KrylovOpts(C.Structure):
 class NewtonOpts(C.Structure):
     _fields_ = [
         ("max_iters", C.c_int32), ("nl_tol", C.c_double), ("backtracking", C.c_int32), ("flags", C.c_int32),
-        ("linear", KrylovOpts),
+        ("linear", KrylovOpts), ("relative_off", C.c_int32), ("use_abs_tol", C.c_int32), ("abs_tol", C.c_double),
     ]
 
 

@@ -281,6 +281,7 @@ static int newton(Ctx& c, double* u, const mhx_newton_opts& o, mhx_solve_stats* 
   bool proceed = true;
   const int maxit = o.max_iters > 0 ? o.max_iters : 10;
   const double tol = o.nl_tol > 0 ? o.nl_tol : 1e-6;
+  const double abstol = o.abs_tol > 0 ? o.abs_tol : std::min(tol, 1.0e-6);
   while (proceed) {
     assemble_physics(c, u, c.res.p, nullptr, true, false, false);  // assembleRes (ScalarT)
     const double nrm = norm_inf(c, c.res.p, n);
@@ -301,7 +302,12 @@ static int newton(Ctx& c, double* u, const mhx_newton_opts& o, mhx_solve_stats* 
       solve = false;
       alpha *= 0.5;
       axpy(c, n, -alpha, du.p, u);
-    } else if (scaled < tol || nrm < 1.0e-100) {
+    } else if (!o.relative_off) {
+      if (scaled < tol || nrm < 1.0e-100) {
+        solve = false;
+        proceed = false;
+      }
+    } else if (o.use_abs_tol && nrm < abstol) {
       solve = false;
       proceed = false;
     }

@@ -177,7 +177,11 @@ struct mhx_server {
         if (slot.ctx && r->status != MHX_ERR_DEVICE) {
           slot.last_used = r->t_end;
           idle.push_back(slot);
-          while ((int)idle.size() > d.max_cached) {  // keep at most max_cached idle contexts
+          // keep at most max_cached idle contexts, trimmed once the queue has
+          // drained: destroying a context frees device memory, which
+          // synchronises the device and would stall every running request
+          // (under memory pressure admit() evicts as needed)
+          while (queue.empty() && (int)idle.size() > d.max_cached) {
             used -= idle.front().bytes;
             drop.push_back(idle.front().ctx);
             idle.pop_front();

@@ -296,11 +296,16 @@ class Mhx:
         return sp.csr_matrix((vals, ci, rp), shape=(rows, cols)), meta
 
     def nonlinearSolver(self, u, max_iters: int = 10, nl_tol: float = 1e-6, backtracking: bool = True,
-                        linear: Optional[capi.KrylovOpts] = None, linear_max_iters: int = 1000, **kw):
+                        linear: Optional[capi.KrylovOpts] = None, linear_max_iters: int = 1000,
+                        use_relative_tol: bool = True, use_absolute_tol: bool = False, abs_tol: float = 0.0,
+                        **kw):
         """Newton on u in place ("max nonlinear iters", "nonlinear TOL", "allow
-        backtracking"); `linear` / keyword arguments set the Krylov solve."""
+        backtracking", "use relative TOL", "use absolute TOL", "absolute
+        nonlinear TOL" with 0 meaning min(nl_tol, 1e-6)); `linear` / keyword
+        arguments set the Krylov solve."""
         o = capi.NewtonOpts()
         o.max_iters, o.nl_tol, o.backtracking = int(max_iters), float(nl_tol), int(backtracking)
+        o.relative_off, o.use_abs_tol, o.abs_tol = int(not use_relative_tol), int(use_absolute_tol), float(abs_tol)
         o.linear = linear or krylov_opts(max_iters=linear_max_iters, **kw)
         pu, du = _ptr(u)
         o.flags = capi.MHX_DEVICE_PTRS if du else 0

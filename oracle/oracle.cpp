@@ -1179,8 +1179,13 @@ static int newton(const Problem& P, double* u, const mhx_newton_opts* o, mhx_sol
       solve = false;
       alpha *= 0.5;
       for (long i = 0; i < n; ++i) u[i] -= alpha * du[i];
-    } else if (scaled < o->nl_tol || nrm < 1e-100) {
-      solve = false;
+    } else if (!o->relative_off) {  // "use relative TOL" (solverManager_solvers.hpp:431-439)
+      if (scaled < o->nl_tol || nrm < 1e-100) {
+        solve = false;
+        proceed = false;
+      }
+    } else if (o->use_abs_tol && nrm < (o->abs_tol > 0 ? o->abs_tol : std::min(o->nl_tol, 1e-6))) {
+      solve = false;  // "use absolute TOL" (:441-443)
       proceed = false;
     }
     if (solve) {

@@ -197,6 +197,50 @@ def test_newton_nonlinear_quadratic_convergence():
     assert norms[3] < norms[2] ** 1.5 / max(norms[0], 1) * 10 or norms[3] < 1e-12
 
 
+def _newton_trace():
+    """The nonlinear problem, and ||F(u_j)||_inf after j = 0..6 Newton updates."""
+    rng = np.random.default_rng(8)
+    prob = Problem(dim=2, order=2, quadrature=4, n=(4, 4), diff_u2=1.0, k0=np.exp(0.5 * rng.standard_normal(16)),
+                   src_elem=rng.uniform(0, 20, 16), dbc_sides=1 | 2)
+    o = Oracle(prob)
+    u0 = o.setDirichlet(np.zeros(o.n_dof))
+    norms = [np.abs(o.assembleRes(u0)).max()]
+    for k in range(1, 7):
+        u, _ = o.nonlinearSolver(u0.copy(), nopts(max_iters=k, nl_tol=1e-30, backtracking=0, tol=1e-14))
+        norms.append(np.abs(o.assembleRes(u)).max())
+    return o, u0, norms
+
+
+def test_newton_absolute_tolerance():
+    """"use relative TOL" false with "use absolute TOL" (solverManager_
+    solvers.hpp:430-445): Newton stops at the first iterate whose residual
+    inf-norm is below the absolute tolerance, found here from the
+    per-iteration norms of separate runs; without either test it runs
+    "max nonlinear iters" steps; the default absolute tolerance is
+    min(nonlinear TOL, 1e-6)."""
+    o, u0, norms = _newton_trace()
+    assert all(norms[j + 1] < norms[j] for j in range(1, 5))  # (the first full step overshoots)
+    for j in (2, 3, 4):
+        tol = np.sqrt(norms[j - 1] * norms[j])  # between iterates j-1 and j, below every earlier one
+        assert min(norms[:j]) > tol
+        opts = nopts(max_iters=10, nl_tol=1e-30, backtracking=0, tol=1e-14)
+        opts.relative_off, opts.use_abs_tol, opts.abs_tol = 1, 1, tol
+        u, st = o.nonlinearSolver(u0.copy(), opts)
+        assert st["nl_iters"] == j + 1
+        assert st["res_norm"] == pytest.approx(norms[j], rel=1e-12)
+    # neither test: exactly max_iters steps although the residual is tiny
+    opts = nopts(max_iters=6, nl_tol=1.0, backtracking=0, tol=1e-14)
+    opts.relative_off = 1
+    _, st = o.nonlinearSolver(u0.copy(), opts)
+    assert st["nl_iters"] == 6
+    # default absolute tolerance min(nl_tol, 1e-6), not nl_tol
+    opts = nopts(max_iters=10, nl_tol=1.0, backtracking=0, tol=1e-14)
+    opts.relative_off, opts.use_abs_tol = 1, 1
+    _, st = o.nonlinearSolver(u0.copy(), opts)
+    j = next(k for k, v in enumerate(norms) if v < 1e-6)
+    assert st["nl_iters"] == j + 1
+
+
 # --- navier stokes ----------------------------------------------------------------------
 from conftest import field_l2_errors, ns_channel_problem  # noqa: E402
 from mrhyde_amd.abi import MHX_PHYS_NAVIER_STOKES  # noqa: E402

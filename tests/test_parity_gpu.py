@@ -288,6 +288,25 @@ def test_newton_parity_nonlinear(pairs):
     assert np.abs(o.assembleRes(ug)).max() <= 1e-9 * sg["res_norm0"]
 
 
+def test_newton_absolute_tolerance_matches_oracle():
+    """The absolute nonlinear tolerance on the GPU stops at the oracle's
+    iteration, with the oracle's iterate."""
+    from mrhyde_amd import Mhx
+    from test_oracle import _newton_trace
+
+    o, u0, norms = _newton_trace()
+    g = Mhx(o.problem)
+    tol = np.sqrt(norms[1] * norms[2])
+    opts = nopts(max_iters=10, nl_tol=1e-30, backtracking=0, tol=1e-14)
+    opts.relative_off, opts.use_abs_tol, opts.abs_tol = 1, 1, tol
+    uo, so = o.nonlinearSolver(u0.copy(), opts)
+    ug, sg = g.nonlinearSolver(u0.copy(), max_iters=10, nl_tol=1e-30, backtracking=False, use_relative_tol=False,
+                               use_absolute_tol=True, abs_tol=tol, linear=opts.linear)
+    assert sg["nl_iters"] == so["nl_iters"] == 3
+    np.testing.assert_allclose(ug, uo, rtol=0, atol=1e-10 * np.abs(uo).max())
+    g.close()
+
+
 GOLD = json.load(open(os.path.join(GOLDEN, "thermal_gold.json")))
 
 
