@@ -369,13 +369,15 @@ int  mhx_l2_error(mhx_ctx* ctx, const double* u, int32_t n_modes, const double* 
  *                         block colour, element colours as in-workgroup
  *                         sub-phases), 2 = x-segments (a workgroup merges a
  *                         run of x-adjacent elements in LDS and writes each
- *                         row's share as contiguous runs; thermal only),
+ *                         row's share as contiguous runs; Navier-Stokes:
+ *                         segments of 8 (3-D) / 16 (2-D) elements; block
+ *                         colouring runs element colouring there),
  *                         3 = pencil streaming (a wavefront walks an
  *                         x-segment of an element pencil two elements per
  *                         step and writes each row's x-run once; 3-D Q2
  *                         thermal with 3x3x3 Gauss on an unperturbed box
  *                         mesh only, MHX_ERR_ARG elsewhere).  Default: 3
- *                         where it applies, else 0.
+ *                         where it applies, 2 for Navier-Stokes, else 0.
  *   "assembly.segment"    pencil streaming: elements per x-segment, even,
  *                         2 .. 4096 (default 12).
  *   "krylov.spmv_lanes"   lanes that share one CRS row in the sparse

@@ -383,6 +383,9 @@ int mhx_ctx_create(const mhx_problem_desc* desc, mhx_ctx** out) {
   // thermal assembly then defaults to the pencil-streamed kernel
   c->box_mesh = desc->perturb == 0.0;
   if (q2box_applicable(*c)) c->scatter_mode = kPencilStream;
+  // Navier-Stokes: x-segments merged in LDS (each row's x-runs written once
+  // per segment) by default
+  if (desc->physics == MHX_PHYS_NAVIER_STOKES) c->scatter_mode = kSegmentMerge;
   c->desc.k0 = nullptr;
   c->desc.src_elem = nullptr;
   c->desc.dbc_values = nullptr;

@@ -220,7 +220,201 @@ __device__ __forceinline__ void ns_tangent(const double* o, const NsCoef& nc, do
 // Jacobian column (n, F) from the tangent of the point residual for the seed
 // of basis n in field F; the field is uniform in the wavefront, so each
 // wavefront runs the tangent specialised for its field.
+// Unit of the segment schedule: x-segment sx of EPW elements of the element
+// row (ey, ez), one colour per (sx, ey, ez) parity; launch order is colour
+// order, so along each axis the even unit of two sharing a vertex plane runs
+// first.
+struct NsSeg {
+  int sx, ey, ez, x0, n;         // segment, pencil, first element, active elements
+  bool flx, fhx, fly, fhy, flz, fhz;  // first toucher of the shared low / high plane
+};
+template <int DIM>
+__device__ __forceinline__ NsSeg ns_seg(const AsmArgs& a, int S) {
+  NsSeg u;
+  int t = blockIdx.x;
+  u.sx = 2 * (t % a.cn[0]) + (a.color & 1);
+  t /= a.cn[0];
+  u.ey = 2 * (t % a.cn[1]) + ((a.color >> 1) & 1);
+  t /= a.cn[1];
+  u.ez = DIM == 3 ? 2 * t + ((a.color >> 2) & 1) : 0;
+  u.x0 = u.sx * S;
+  u.n = min(S, a.m.nel[0] - u.x0);
+  u.flx = u.sx == 0 || (u.sx & 1) == 0;
+  u.fhx = u.sx == a.nseg - 1 || (u.sx & 1) == 0;
+  u.fly = u.ey == 0 || (u.ey & 1) == 0;
+  u.fhy = u.ey == a.m.nel[1] - 1 || (u.ey & 1) == 0;
+  u.flz = DIM < 3 || u.ez == 0 || (u.ez & 1) == 0;
+  u.fhz = DIM < 3 || u.ez == a.m.nel[2] - 1 || (u.ez & 1) == 0;
+  return u;
+}
+
+template <int DIM, bool SEG>
+__device__ __forceinline__ bool ns_element(const AsmArgs& a, const NsSeg& u, int e, int* ec) {
+  if constexpr (SEG) {
+    ec[0] = u.x0 + (e < u.n ? e : 0);
+    ec[1] = u.ey;
+    ec[2] = u.ez;
+    return e < u.n;
+  } else {
+    return element_of<DIM>(a, e, 0, ec);
+  }
+}
+
+// Scatter of an x-segment: element el (lanes of every wavefront, column (jn,
+// F) per lane) puts its rows into LDS, even elements first (stores), then odd
+// ones (adding where the entry lies on a plane shared with an even
+// neighbour), so an entry on a plane shared inside the segment is the sum of
+// exactly two contributions in a fixed order; each row's x-runs are then
+// read back (if an earlier unit wrote them) and written once.  One round per
+// row field (the LDS buffer is the point data's); the per-entry addressing
+// and first-touch flags, which do not depend on the row field, are formed
+// once.  An entry is shared with another unit only on the segment's end
+// planes (x) or on a (y, z) vertex plane of the pencil; the first scheduled
+// unit among the sharers stores it.
+// Read-backs of one round: the CRS address of each of the lane's entries in
+// the rows of field fr, and the value an earlier unit stored there (0 where
+// this unit is the first toucher, on Dirichlet rows and for unused slots).
+template <int NIT, int NF>
+__device__ __forceinline__ void ns_seg_readback(const AsmArgs& a, const int* s_rps, const int* it_r,
+                                                const int* it_off, const unsigned* it_f, int fr, long* addr,
+                                                double* old, bool* fixd) {
+#pragma unroll
+  for (int k = 0; k < NIT; ++k) {
+    int rp = s_rps[it_r[k] * NF + fr];
+    fixd[k] = rp < 0;
+    if (fixd[k]) rp = -1 - rp;
+    addr[k] = (long)rp + it_off[k];
+    const unsigned f = it_f[k];
+    old[k] = ((f & 1u) && !fixd[k] && !(f & 2u)) ? a.jac[addr[k]] : 0.0;
+  }
+}
+
 template <int DIM, bool JAC>
+__device__ __forceinline__ void ns_seg_scatter(const AsmArgs& a, const NsSeg& u, double* buf, const int* s_rps,
+                                               bool active, int el, int jn, int wv, int t, double rres,
+                                               const double* R) {
+  using C = NsCfg<DIM>;
+  constexpr int NF = C::NF, NV = C::NV, EPW = C::EPW, NT = C::NT;
+  constexpr int YZ = 1 << (DIM - 1), RPS = (EPW + 1) * YZ, CPR = YZ * 3 * NF;
+  constexpr int NJ = JAC ? RPS * CPR : 0;  // Jacobian entries of a round
+  constexpr int NIT = JAC ? (NJ + NT - 1) / NT : 1;
+  static_assert(NJ + RPS <= EPW * C::NQ * C::NQS, "a round fits over the point data");
+  double* jb = buf;       // [RPS][CPR]: row (x plane, yz line), column ((yz line) 3 + x offset) NF + field
+  double* rb = buf + NJ;  // [RPS]: residual rows of the round's field
+  const int jx = jn & 1, jyz = jn >> 1;
+  const int X0 = u.x0, X1 = u.x0 + u.n;
+  const int nn0 = a.m.nn[0], nn1 = a.m.nn[1];
+  // this lane's entries q = t + k NT: row r, offset in the row's CRS box,
+  // flags 1 write, 2 first, 4 diagonal node pair, field of the column << 4
+  int it_r[NIT], it_off[NIT];
+  unsigned it_f[NIT];
+#pragma unroll
+  for (int k = 0; k < NIT; ++k) {
+    it_r[k] = 0;
+    it_off[k] = 0;
+    it_f[k] = 0;
+    if constexpr (JAC) {
+      const int q = t + k * NT;
+      const int r = q / CPR, c = q - (q / CPR) * CPR;
+      const int xp = r / YZ, yz = r % YZ, cyz = c / (3 * NF), cx = (c / NF) % 3, fc = c % NF;
+      const int X = X0 + xp, Xc = X + cx - 1;
+      const bool valid = q < NJ && xp <= u.n && Xc >= X0 && Xc <= X1;
+      const int jy = yz & 1, jz = yz >> 1, cy = cyz & 1, cz = cyz >> 1;
+      const int Y = u.ey + jy, Z = u.ez + jz, Yc = u.ey + cy, Zc = u.ez + cz;
+      const int lox = X > 0 ? X - 1 : 0, loy = Y > 0 ? Y - 1 : 0, loz = Z > 0 ? Z - 1 : 0;
+      const int lx = (X + 1 < nn0 ? X + 1 : nn0 - 1) - lox + 1;
+      const int ly = (Y + 1 < nn1 ? Y + 1 : nn1 - 1) - loy + 1;
+      const int pz = DIM == 3 ? Zc - loz : 0;
+      const bool xsh = X == Xc && ((X == X0 && X0 > 0) || (X == X1 && X1 < a.m.nel[0]));
+      const bool xr = !xsh || (X == X0 ? u.flx : u.fhx);
+      const bool yr = Y != Yc || (jy == 0 ? u.fly : u.fhy);
+      const bool zr = DIM < 3 || Z != Zc || (jz == 0 ? u.flz : u.fhz);
+      const bool first = a.first_touch && xr && yr && zr;
+      const bool dnode = X == Xc && Y == Yc && Z == Zc;
+      it_r[k] = valid ? r : 0;
+      it_off[k] = ((pz * ly + (Yc - loy)) * lx + (Xc - lox)) * NF + fc;
+      it_f[k] = (valid ? 1u : 0u) | (first ? 2u : 0u) | (dnode ? 4u : 0u) | ((unsigned)fc << 4);
+    }
+  }
+  // residual row t (< RPS) of each round: node and first-touch flag
+  long res_node = 0;
+  bool res_go = false, res_first = false;
+  if (t < RPS) {
+    const int xp = t / YZ, yz = t % YZ, jy = yz & 1, jz = yz >> 1;
+    res_go = a.res != nullptr && xp <= u.n;
+    res_node = ((long)(u.ez + jz) * nn1 + (u.ey + jy)) * nn0 + (X0 + xp);
+    const bool xr = xp == 0 ? u.flx : (xp == u.n ? u.fhx : true);
+    res_first = a.first_touch && xr && (jy == 0 ? u.fly : u.fhy) && (DIM < 3 || (jz == 0 ? u.flz : u.fhz));
+  }
+  // read-backs of round 0 (later touches of entries of non-Dirichlet rows)
+  long addr[NIT], addr_n[NIT];
+  double old[NIT], old_n[NIT];
+  bool fixd[NIT], fix_n[NIT];
+  if (JAC && a.jac) ns_seg_readback<NIT, NF>(a, s_rps, it_r, it_off, it_f, 0, addr, old, fixd);
+  // odd element: its planes shared with the even neighbours (low: el - 1;
+  // high: el + 1 when it exists)
+  const bool odd = el & 1;
+  const bool sh_lo = odd, sh_hi = odd && el + 1 < u.n;
+  __syncthreads();  // every wavefront is done with the point data
+#pragma unroll
+  for (int fr = 0; fr < NF; ++fr) {
+#pragma unroll
+    for (int par = 0; par < 2; ++par) {
+      if (active && (el & 1) == par) {
+        if (wv == fr) {
+          double* p = &rb[(el + jx) * YZ + jyz];
+          const bool add = jx == 0 ? sh_lo : sh_hi;
+          *p = add ? *p + rres : rres;
+        }
+        if constexpr (JAC) {
+#pragma unroll
+          for (int in = 0; in < NV; ++in) {
+            const int ix = in & 1;
+            double* p = &jb[((el + ix) * YZ + (in >> 1)) * CPR + (jyz * 3 + (jx - ix + 1)) * NF + wv];
+            const bool add = ix == jx && (ix == 0 ? sh_lo : sh_hi);
+            *p = add ? *p + R[in * NF + fr] : R[in * NF + fr];
+          }
+        }
+      }
+      __syncthreads();
+    }
+    if (res_go) {
+      const long row = res_node * NF + fr;
+      const bool fx = s_rps[t * NF + fr] < 0;
+      const double v = -rb[t];
+      if (res_first) a.res[row] = fx ? 0.0 : v;
+      else if (!fx) a.res[row] += v;
+    }
+    if constexpr (JAC) {
+      if (a.jac) {
+        // the next round's read-backs go out before this round's stores
+        // (disjoint rows; loads issued ahead of stores do not wait for them)
+        if (fr + 1 < NF) ns_seg_readback<NIT, NF>(a, s_rps, it_r, it_off, it_f, fr + 1, addr_n, old_n, fix_n);
+#pragma unroll
+        for (int k = 0; k < NIT; ++k) {
+          const unsigned f = it_f[k];
+          if (!(f & 1u)) continue;
+          const bool first = f & 2u;
+          if (fixd[k]) {
+            if (first) a.jac[addr[k]] = ((f & 4u) && (int)(f >> 4) == fr) ? 1.0 : 0.0;
+          } else {
+            const double v = jb[t + k * NT];
+            a.jac[addr[k]] = first ? v : old[k] + v;
+          }
+        }
+#pragma unroll
+        for (int k = 0; k < NIT; ++k) {
+          addr[k] = addr_n[k];
+          old[k] = old_n[k];
+          fixd[k] = fix_n[k];
+        }
+      }
+    }
+    __syncthreads();  // the next round overwrites the buffer
+  }
+}
+
+template <int DIM, bool JAC, bool SEG>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_ns(AsmArgs a, NsCoef nc) {
   using C = NsCfg<DIM>;
   constexpr int NV = C::NV, NQ = C::NQ, NF = C::NF, NE = C::NE, EPW = C::EPW, NQS = C::NQS;
@@ -228,27 +422,32 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   __shared__ double s_X[EPW][NV * DIM];
   __shared__ double s_q[EPW][NQ][NQS];
   __shared__ int s_rp[EPW][NE];
+  constexpr int YZ = 1 << (DIM - 1);      // (y, z) node lines of an element row
+  constexpr int RPS = (EPW + 1) * YZ;     // segment: node rows per field
+  __shared__ int s_rps[SEG ? RPS * NF : 1];  // segment: CRS offset of row (x plane, yz line, field)
 
   const int t = threadIdx.x;
   const int wv = __builtin_amdgcn_readfirstlane(t / 64), lane = t % 64;
   const int el = lane / NV;  // element of this lane in the workgroup (column and point stages)
+  const NsSeg u = SEG ? ns_seg<DIM>(a, EPW) : NsSeg{};
 
   // 1. gather: element unknowns (node-major, field-minor), CRS offsets, vertices
   for (int k = t; k < EPW * NE; k += C::NT) {
     const int e = k / NE, j = k % NE;
     int ec[3];
-    const bool act = element_of<DIM>(a, e, 0, ec);
+    const bool act = ns_element<DIM, SEG>(a, u, e, ec);
     if (act) {
       const int n = j / NF, f = j % NF;
       const long dof = node_of<DIM, 1>(a.m, ec, n) * NF + f;
       s_u[e][j] = a.u[dof];
       s_rp[e][j] = a.fixed[dof] ? -1 - a.rowptr[dof] : a.rowptr[dof];
+      if constexpr (SEG) s_rps[((e + (n & 1)) * YZ + (n >> 1)) * NF + f] = s_rp[e][j];  // shared planes: same value
     }
   }
   for (int k = t; k < EPW * NV * DIM; k += C::NT) {
     const int e = k / (NV * DIM), r = k % (NV * DIM);
     int ec[3];
-    if (element_of<DIM>(a, e, 0, ec)) {
+    if (ns_element<DIM, SEG>(a, u, e, ec)) {
       const int v = r / DIM, d = r % DIM;
       const long vid = node_of<DIM, 1>(a.m, ec, v);  // Q1: nodes are the vertices
       s_X[e][r] = a.vx[vid * 3 + d];
@@ -258,7 +457,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 
   // 2. point stage, part 1: geometry and fields (wave 0, lane = (element, point))
   int ecp[3];
-  const bool pact = wv == 0 && element_of<DIM>(a, el, 0, ecp);
+  const bool pact = wv == 0 && ns_element<DIM, SEG>(a, u, el, ecp);
   const int q = lane % NQ;
   if (pact) {
     double J[DIM][DIM];
@@ -437,7 +636,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   // 3. column stage: wave F, lane (element el, node jn): residual row and
   // Jacobian column (jn, F)
   int ec[3];
-  const bool active = wv < NF && element_of<DIM>(a, el, 0, ec);
+  const bool active = wv < NF && ns_element<DIM, SEG>(a, u, el, ec);
   const int F = wv, jn = lane % NV, jl = jn * NF + F;
   double R[JAC ? NE : 1];
 #pragma unroll
@@ -485,6 +684,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         }
       }
     }
+  }
+
+  if constexpr (SEG) {
+    ns_seg_scatter<DIM, JAC>(a, u, &s_q[0][0][0], s_rps, active, el, jn, wv, t, rres, R);
+    return;
   }
 
   // 4. scatter: first scheduled element touching an entry stores it
@@ -570,31 +774,40 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   }
 }
 
+// Element colouring (one launch per element colour, EPW elements of the
+// colour per workgroup) or, with seg, x-segments of EPW consecutive elements
+// per workgroup, one launch per (segment, ey, ez) parity.
 template <int DIM>
-static void launch_ns(Ctx& c, AsmArgs& a, const NsCoef& nc, bool jac) {
+static void launch_ns(Ctx& c, AsmArgs& a, const NsCoef& nc, bool jac, bool seg) {
   using C = NsCfg<DIM>;
   const int ncol = 1 << DIM;
   a.gpb = C::EPW;
   a.blocked = 0;
   a.nsub = 1;
+  a.nseg = (c.mesh.nel[0] + C::EPW - 1) / C::EPW;
   for (int col = 0; col < ncol; ++col) {
     a.color = col;
     long cnt = 1;
     for (int d = 0; d < 3; ++d) {
       const int cd = (col >> d) & 1;
-      int n = d < DIM ? (c.mesh.nel[d] - cd + 1) / 2 : 1;
+      const int units = d == 0 && seg ? a.nseg : (d < DIM ? c.mesh.nel[d] : 1);
+      int n = d < DIM ? (units - cd + 1) / 2 : 1;
       if (d >= DIM && cd) n = 0;
       a.cn[d] = n;
       cnt *= n;
     }
     a.ncolor = cnt;
     if (cnt == 0) continue;
-    const long blocks = (cnt + C::EPW - 1) / C::EPW;
+    const long blocks = seg ? cnt : (cnt + C::EPW - 1) / C::EPW;
     ScopedPhase ph(c.prof, jac ? "k_ns<jacres>" : "k_ns<res>", c.stream);
-    if (jac)
-      hipLaunchKernelGGL((k_ns<DIM, true>), dim3((unsigned)blocks), dim3(C::NT), 0, c.stream, a, nc);
-    else
-      hipLaunchKernelGGL((k_ns<DIM, false>), dim3((unsigned)blocks), dim3(C::NT), 0, c.stream, a, nc);
+    const dim3 grid((unsigned)blocks), block(C::NT);
+    if (seg) {
+      if (jac) hipLaunchKernelGGL((k_ns<DIM, true, true>), grid, block, 0, c.stream, a, nc);
+      else hipLaunchKernelGGL((k_ns<DIM, false, true>), grid, block, 0, c.stream, a, nc);
+    } else {
+      if (jac) hipLaunchKernelGGL((k_ns<DIM, true, false>), grid, block, 0, c.stream, a, nc);
+      else hipLaunchKernelGGL((k_ns<DIM, false, false>), grid, block, 0, c.stream, a, nc);
+    }
     MHX_HIP(hipGetLastError());
   }
 }
@@ -635,8 +848,9 @@ void assemble_ns(Ctx& c, const double* u, double* res, double* jac, bool want_re
   nc.pspg = c.desc.ns_pspg;
   {
     ScopedPhase ph(c.prof, want_jac ? "AssemblyManager::assembleJacRes" : "AssemblyManager::assembleRes", c.stream);
-    if (c.mesh.dim == 2) launch_ns<2>(c, a, nc, want_jac);
-    else launch_ns<3>(c, a, nc, want_jac);
+    const bool seg = c.scatter_mode == kSegmentMerge;
+    if (c.mesh.dim == 2) launch_ns<2>(c, a, nc, want_jac, seg);
+    else launch_ns<3>(c, a, nc, want_jac, seg);
   }
   if (accumulate && want_jac && !c.h_dbc_list.empty()) {
     ScopedPhase ph(c.prof, "AssemblyManager::dofConstraints", c.stream);
