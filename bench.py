@@ -322,6 +322,7 @@ def c5_requests(count: int, seed: int):
 
 
 CACHED = 256  # idle contexts a server keeps (within its HBM budget)
+LDS_PEAK_TBS = 78.6  # MI355X LDS: 128 B / clock / CU x 256 CUs x 2.4 GHz
 
 
 def serving_bench(n_req: int = 240, rate: float = 200.0, seed: int = 5, per_call_sample: int = 12):
@@ -437,6 +438,14 @@ def subgrid_bench(steps: int, per_call_sample: int = 16):
                      "node_iterations_per_s": round(node_its / (t * 1e-3), 1),
                      "est_tflops": round(154.0 * node_its / (t * 1e-3) / 1e12, 2),
                      "est_lds_TBps": round((35.0 * (8 if prec == "fp64" else 4)) * node_its / (t * 1e-3) / 1e12, 2)}
+        # the two ceilings of the on-chip solve: vector FP64 (FP32 inner:
+        # the FP32 vector rate) and LDS bandwidth (128 B per clock per CU,
+        # 256 CUs at 2.4 GHz = 78.6 TB/s)
+        flop_peak = 78.6 if prec == "fp64" else 157.3
+        out[prec]["roofline"] = {"bound": "lds", "achieved": out[prec]["est_lds_TBps"], "peak": LDS_PEAK_TBS,
+                                 "unit": "TB/s", "frac": round(out[prec]["est_lds_TBps"] / LDS_PEAK_TBS, 4),
+                                 "flop_frac": round(out[prec]["est_tflops"] / flop_peak, 4),
+                                 "flop_peak_TFLOPs": flop_peak}
         b.close()
     # plain per-call path: one engine context + J/R assembly + Jacobi-CG per instance
     ko = np.concatenate([[0], np.cumsum(n_axis.astype(np.int64) ** 3)])

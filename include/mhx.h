@@ -90,7 +90,12 @@ typedef struct mhx_problem_desc {
   /* Neumann conditions (thermal boundaryResidual, "Neumann",
    * src/physics/thermal.cpp:210-222): on the sides in neumann_sides the
    * outward flux g = q(x) . n enters as res += int_side g phi, with
-   * q_c(x) = sum_m nq_amp[m][c] * prod_d sin(nq_freq[m][d] x_d + nq_phase[m][d]). */
+   * q_c(x) = sum_m nq_amp[m][c] * prod_d sin(nq_freq[m][d] x_d + nq_phase[m][d]).
+   * Navier-Stokes (navierstokes boundaryResidual, traction, "Neumann u<c>",
+   * src/physics/navierstokes.cpp:853-1000): q is the traction vector and
+   * velocity component c gets res_c += int_side q_c phi on those sides
+   * (where component c is not Dirichlet); a phase of pi/2 with frequency 0
+   * gives a constant. */
   uint32_t neumann_sides;
   int32_t  n_nq_modes;    /* <= MHX_MAX_NQ_MODES */
   double  nq_amp[MHX_MAX_NQ_MODES][3];

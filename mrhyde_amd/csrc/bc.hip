@@ -1,10 +1,15 @@
-// bc.hip — boundary residual terms of the thermal module on Neumann sides.
+// bc.hip — boundary residual terms on Neumann sides: the thermal flux and
+// the Navier-Stokes traction.
 //
 // Replaces thermal<EvalT>::boundaryResidual for "Neumann" sides
-// (src/physics/thermal.cpp:210-222) and the boundary-group half of
+// (src/physics/thermal.cpp:210-222), navierstokes<EvalT>::boundaryResidual
+// for "Neumann" (traction) sides of the velocity components
+// (src/physics/navierstokes.cpp:853-1000), and the boundary-group half of
 // AssemblyManager::assembleJacRes (assemblyManager_jacres.hpp:389-401):
-// res_glob[row] += sum over side points of g w_side phi_row, g = q(x) . n.
-// g does not depend on u, so the Jacobian is unchanged.
+//   thermal:        res_glob[row] += sum over side points of g w_side phi_row, g = q(x) . n;
+//   Navier-Stokes:  res_glob[node, c] += sum over side points of q_c(x) w_side phi_node,
+//                   c = velocity component (the reference's "Neumann u<c> <side>").
+// The data do not depend on u, so the Jacobian is unchanged.
 //
 // One launch per Neumann side, one thread per lattice node of that side: the
 // thread sums the contributions of the (at most 2^(dim-1)) boundary faces
@@ -26,6 +31,7 @@ struct NeumannArgs {
   const double* vx;
   const uint8_t* fixed;
   double* res;
+  int nf;    // unknowns per node: 1 thermal (flux q . n), dim + 1 Navier-Stokes (traction q)
   int side;  // 2*axis + (high side)
   int n_terms;
   double amp[MHX_MAX_NQ_MODES][3], freq[MHX_MAX_NQ_MODES][3], phase[MHX_MAX_NQ_MODES][3];
@@ -48,7 +54,10 @@ __global__ __launch_bounds__(256) void k_neumann(NeumannArgs a) {
   if (NT == 2) c[ta[1]] = (int)(tid / a.m.nn[ta[0]]);
   if (DIM == 2) c[2] = 0;
   const long node = c[0] + (long)a.m.nn[0] * (c[1] + (long)a.m.nn[1] * c[2]);
-  if (a.fixed[node]) return;
+  const int ncomp = a.nf == 1 ? 1 : DIM;  // residual rows of the node this side loads
+  bool any = false;
+  for (int k = 0; k < ncomp; ++k) any = any || !a.fixed[node * a.nf + k];
+  if (!any) return;
   // candidate elements along each tangential axis
   int e0[2], ne_t[2];
   for (int t = 0; t < NT; ++t) {
@@ -62,9 +71,9 @@ __global__ __launch_bounds__(256) void k_neumann(NeumannArgs a) {
       ne_t[t] = 1;
     }
   }
-  double sum = 0.0;
-  const int nf = NT == 1 ? ne_t[0] : ne_t[0] * ne_t[1];
-  for (int f = 0; f < nf; ++f) {
+  double sum[3] = {0.0, 0.0, 0.0};
+  const int nfc = NT == 1 ? ne_t[0] : ne_t[0] * ne_t[1];
+  for (int f = 0; f < nfc; ++f) {
     int ec[3];
     ec[ax] = hi ? a.m.nel[ax] - 1 : 0;
     ec[ta[0]] = e0[0] + f % ne_t[0];
@@ -117,28 +126,35 @@ __global__ __launch_bounds__(256) void k_neumann(NeumannArgs a) {
       }
       if (nrm[ax] * (hi ? 1.0 : -1.0) < 0.0)
         for (int d = 0; d < 3; ++d) nrm[d] = -nrm[d];
-      double g = 0.0;
+      double g[3] = {0.0, 0.0, 0.0};
       for (int m = 0; m < a.n_terms; ++m) {
         double prod = 1.0;
         for (int d = 0; d < DIM; ++d) prod *= sin(a.freq[m][d] * x[d] + a.phase[m][d]);
-        for (int d = 0; d < DIM; ++d) g += a.amp[m][d] * prod * nrm[d];
+        if (a.nf == 1) {
+          for (int d = 0; d < DIM; ++d) g[0] += a.amp[m][d] * prod * nrm[d];
+        } else {
+          for (int d = 0; d < DIM; ++d) g[d] += a.amp[m][d] * prod;
+        }
       }
-      sum = fma(g * w * area, phi, sum);
+      for (int k = 0; k < ncomp; ++k) sum[k] = fma(g[k] * w * area, phi, sum[k]);
     }
   }
-  a.res[node] += sum;
+  for (int k = 0; k < ncomp; ++k)
+    if (!a.fixed[node * a.nf + k]) a.res[node * a.nf + k] += sum[k];
 }
 
 void neumann_residual(Ctx& c, double* res) {
   const mhx_problem_desc& d = c.desc;
   if (!res || !d.neumann_sides || d.n_nq_modes <= 0) return;
-  ScopedPhase ph(c.prof, "thermal::boundaryResidual", c.stream);
+  ScopedPhase ph(c.prof, c.mesh.nf == 1 ? "thermal::boundaryResidual" : "navierstokes::boundaryResidual",
+                 c.stream);
   NeumannArgs a{};
   a.m = c.mesh;
   a.t = c.tab;
   a.vx = c.vx.p;
   a.fixed = c.fixed.p;
   a.res = res;
+  a.nf = c.mesh.nf;
   a.n_terms = d.n_nq_modes;
   for (int m = 0; m < d.n_nq_modes; ++m)
     for (int k = 0; k < 3; ++k) {
@@ -474,7 +490,8 @@ void nitsche_boundary(Ctx& c, const double* u, double* res, double* jac) {
   if (!d.wdbc_sides || (!res && !jac)) return;
   MHX_REQUIRE(nitsche_supported(c.mesh.dim, c.mesh.p), MHX_ERR_UNSUPPORTED,
               "weak Dirichlet sides: no kernel built for this dimension and order");
-  ScopedPhase ph(c.prof, "thermal::boundaryResidual", c.stream);
+  ScopedPhase ph(c.prof, c.mesh.nf == 1 ? "thermal::boundaryResidual" : "navierstokes::boundaryResidual",
+                 c.stream);
   NitscheArgs a{};
   a.m = c.mesh;
   a.t = c.tab;

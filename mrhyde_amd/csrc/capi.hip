@@ -100,8 +100,8 @@ static void validate(const mhx_problem_desc* d) {
     MHX_REQUIRE(d->wdbc_form >= 0 && d->wdbc_form <= 2, MHX_ERR_ARG, "wdbc_form must be 0, 1 or 2");
     MHX_REQUIRE(d->n_wd_modes >= 0 && d->n_wd_modes <= MHX_MAX_NQ_MODES, MHX_ERR_ARG, "n_wd_modes out of range");
   }
-  MHX_REQUIRE(d->neumann_sides == 0 || d->physics == MHX_PHYS_THERMAL, MHX_ERR_UNSUPPORTED,
-              "Neumann sides are built for the thermal module only");
+  MHX_REQUIRE(d->neumann_sides == 0 || d->physics == MHX_PHYS_THERMAL || d->physics == MHX_PHYS_NAVIER_STOKES,
+              MHX_ERR_UNSUPPORTED, "Neumann sides are built for the thermal and Navier-Stokes modules only");
   if (d->physics == MHX_PHYS_NAVIER_STOKES) {
     MHX_REQUIRE(d->ns_rho > 0.0 && d->ns_mu > 0.0, MHX_ERR_ARG, "navier stokes: density and viscosity must be > 0");
     MHX_REQUIRE(ns_supported(d->dim, d->order, d->quadrature / 2 + 1), MHX_ERR_UNSUPPORTED,

@@ -852,6 +852,9 @@ void assemble_ns(Ctx& c, const double* u, double* res, double* jac, bool want_re
     if (c.mesh.dim == 2) launch_ns<2>(c, a, nc, want_jac, seg);
     else launch_ns<3>(c, a, nc, want_jac, seg);
   }
+  // traction sides (navierstokes.cpp:853-1000): the load vector, after the
+  // volume terms (it adds into rows the volume launches wrote)
+  if (want_res) neumann_residual(c, res);
   if (accumulate && want_jac && !c.h_dbc_list.empty()) {
     ScopedPhase ph(c.prof, "AssemblyManager::dofConstraints", c.stream);
     const long n = (long)c.h_dbc_list.size();

@@ -668,11 +668,17 @@ static void neumann_residual(const Problem& P, double* res) {
         const double outward = hi ? 1.0 : -1.0;
         if (nrm[a] * outward < 0) for (int c = 0; c < 3; ++c) nrm[c] = -nrm[c];
         w *= area;
-        double g = 0.0;
+        // thermal: g = q . n (one field); Navier-Stokes: the traction q_c
+        // of each velocity component (navierstokes.cpp:853-1000)
+        const int ncomp = P.nf == 1 ? 1 : dim;
+        double gc[3] = {0.0, 0.0, 0.0};
         for (int m = 0; m < d.n_nq_modes; ++m) {
           double prod = 1.0;
           for (int b = 0; b < dim; ++b) prod *= std::sin(d.nq_freq[m][b] * x[b] + d.nq_phase[m][b]);
-          for (int c = 0; c < dim; ++c) g += d.nq_amp[m][c] * prod * nrm[c];
+          for (int c = 0; c < dim; ++c) {
+            if (P.nf == 1) gc[0] += d.nq_amp[m][c] * prod * nrm[c];
+            else gc[c] += d.nq_amp[m][c] * prod;
+          }
         }
         // basis of every element DOF at the side point
         std::vector<double> Lb(dim * (p + 1));
@@ -684,9 +690,11 @@ static void neumann_residual(const Problem& P, double* res) {
           const int jl[3] = {j % (p + 1), (j / (p + 1)) % (p + 1), dim == 3 ? j / ((p + 1) * (p + 1)) : 0};
           double phi = 1.0;
           for (int b = 0; b < dim; ++b) phi *= Lb[b * (p + 1) + jl[b]];
-          const int row = P.lids[e * nb + j];
-          if (P.fixed[row] || phi == 0.0) continue;
-          res[row] += -(-g * w * phi);
+          for (int c = 0; c < ncomp; ++c) {
+            const int row = P.lids[(e * P.nf + c) * nb + j];
+            if (P.fixed[row] || phi == 0.0) continue;
+            res[row] += -(-gc[c] * w * phi);
+          }
         }
       }
     }
@@ -1019,7 +1027,7 @@ static void assemble(const Problem& P, const double* u, bool want_jac, double* r
       default: { struct T { using type = SFad<64>; }; run(T{}); break; }
     }
   }
-  if (!ns) neumann_residual(P, res);  // boundary groups, after the volume groups
+  neumann_residual(P, res);  // boundary groups, after the volume groups
 }
 
 // ---------------------------------------------------------------------------

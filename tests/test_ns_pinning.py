@@ -260,3 +260,39 @@ def test_ns_kovasznay_rotated_3d_equals_2d():
     for a, b in zip(e2[:3], e3[:3]):
         assert abs(b - s * a) <= 1e-9 * s * a, (a, b)
     assert e3[3][1] < 1e-12  # uy
+
+
+# --- traction (Neumann) sides, navierstokes.cpp:853-1000 ------------------------------
+def _traction_problem(dim, flux, sides, perturb=0.0):
+    n = (4, 3) if dim == 2 else (3, 3, 2)
+    return Problem(dim=dim, order=1, quadrature=2, n=n, physics=MHX_PHYS_NAVIER_STOKES, mu=0.05, rho=1.3,
+                   supg=True, pspg=True, dbc_sides=1, neumann_sides=sides, neumann_flux=flux, perturb=perturb,
+                   perturb_seed=5)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_ns_traction_load_integrates_the_traction(dim):
+    """At u = 0 with no body force every volume term vanishes, so the
+    residual is the traction load alone: summed over the loaded side's rows
+    of component c it is the traction's integral over the side (partition of
+    unity), constant or varying along the side, and the pressure rows and
+    the other sides get nothing."""
+    half_pi = np.pi / 2
+    const = ([0.7, -0.4, 0.25][:dim] + [0.0] * (3 - dim), [0.0] * 3, [half_pi] * 3)  # sin(pi/2) = 1
+    o = Oracle(_traction_problem(dim, [const], 2))  # the x = 1 side
+    r = o.assembleRes(np.zeros(o.n_dof))
+    nf = dim + 1
+    coords = o.dof_coords()[::nf]
+    on = np.isclose(coords[:, 0], 1.0)
+    area = 1.0  # the unit box's x = 1 side
+    for c in range(dim):
+        np.testing.assert_allclose(r[c::nf][on].sum(), const[0][c] * area, rtol=1e-13)
+        assert np.abs(r[c::nf][~on]).max() == 0.0
+    assert np.abs(r[dim::nf]).max() == 0.0
+    # a traction varying along y: the 2-point Gauss rule against the exact integral
+    w = 1.3
+    vary = ([1.0, 0.0, 0.0], [0.0, w, 0.0], [half_pi, 0.2, half_pi])  # q_x = sin(w y + 0.2)
+    o2 = Oracle(_traction_problem(dim, [vary], 2))
+    r2 = o2.assembleRes(np.zeros(o2.n_dof))
+    exact = (np.cos(0.2) - np.cos(w + 0.2)) / w
+    np.testing.assert_allclose(r2[0::nf][on].sum(), exact, rtol=1e-4)

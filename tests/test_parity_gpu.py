@@ -293,11 +293,12 @@ def test_newton_absolute_tolerance_matches_oracle():
     iteration, with the oracle's iterate."""
     from mrhyde_amd import Mhx
     from test_oracle import _newton_trace
+    from test_oracle import nopts as trace_opts
 
     o, u0, norms = _newton_trace()
     g = Mhx(o.problem)
     tol = np.sqrt(norms[1] * norms[2])
-    opts = nopts(max_iters=10, nl_tol=1e-30, backtracking=0, tol=1e-14)
+    opts = trace_opts(max_iters=10, nl_tol=1e-30, backtracking=0, tol=1e-14)
     opts.relative_off, opts.use_abs_tol, opts.abs_tol = 1, 1, tol
     uo, so = o.nonlinearSolver(u0.copy(), opts)
     ug, sg = g.nonlinearSolver(u0.copy(), max_iters=10, nl_tol=1e-30, backtracking=False, use_relative_tol=False,
@@ -415,3 +416,27 @@ def test_iteration_cap_returns(pairs):
     u = o.setDirichlet(np.zeros(o.n_dof))
     _, st = g.nonlinearSolver(u, max_iters=1, nl_tol=1e-30, solver="gmres", tol=1e-10, linear_max_iters=2000)
     assert st["nl_iters"] == 1 and st["status"] == 0
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_ns_traction_sides_match_oracle(dim):
+    """Navier-Stokes with traction on two sides of a perturbed mesh: residual
+    and Jacobian on the GPU against the oracle, both schedules."""
+    from mrhyde_amd import Mhx
+    from oracle import Oracle
+    from test_ns_pinning import _traction_problem
+
+    flux = [([0.7, -0.4, 0.25], [0.0, 1.1, 0.7], [np.pi / 2, 0.3, 0.1]), ([0.2, 0.5, -0.3], [0.9, 0.0, 0.0],
+                                                                          [0.4, np.pi / 2, np.pi / 2])]
+    p = _traction_problem(dim, flux, 2 | 8, perturb=0.15)
+    g, o = Mhx(p), Oracle(p)
+    u = o.setDirichlet(np.random.default_rng(4).uniform(-0.5, 0.5, o.n_dof))
+    ro, jo = o.assembleJacRes(u)
+    rp, _ = o.crs_pattern()
+    for colouring in (2, 0):
+        print(f"[traction] dim={dim} colouring={colouring}", flush=True)
+        g.set_option("assembly.colouring", colouring)
+        rg, jg = g.assembleJacRes(u)
+        assert_entries_close(jg, jo, rp, what=f"traction {dim}-D Jacobian")
+        assert_entries_close(rg, ro, None, what=f"traction {dim}-D residual")
+    g.close()
