@@ -323,6 +323,7 @@ def c5_requests(count: int, seed: int):
 
 CACHED = 256  # idle contexts a server keeps (within its HBM budget)
 LDS_PEAK_TBS = 78.6  # MI355X LDS: 128 B / clock / CU x 256 CUs x 2.4 GHz
+FP64_MFMA_PEAK_TFS = 78.6  # MI355X dense FP64 matrix peak
 
 
 def serving_bench(n_req: int = 240, rate: float = 200.0, seed: int = 5, per_call_sample: int = 12):
@@ -497,6 +498,7 @@ def main():
     ap.add_argument("--ns-n", type=int, default=64, help="Navier-Stokes cavity elements per axis (0 = skip)")
     ap.add_argument("--ns-no-newton", action="store_true", help="Navier-Stokes: assembly only")
     ap.add_argument("--serve-requests", type=int, default=240, help="serving requests (0 = skip)")
+    ap.add_argument("--mfma-probe", type=int, default=1, help="time the matrix-core element contraction (0 = skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: the multi-GPU form "
@@ -564,6 +566,21 @@ def main():
         if capi.lib.mhx_copy_probe(nd, 5, C.byref(ms)) == 0:
             probe = round(16.0 * nd / (ms.value * 1e-3) / 1e9, 1)
 
+    # the dense matrix-core form of the element contraction, on as many
+    # elements as one colour launch of this assembly
+    mfma = None
+    if args.mfma_probe:
+        ne = int(round(g.n_elem / launches_per_step))
+        mfma = {"elements": ne, "mfma_per_element": 108, "assembly_launch_ms": round(mean_launch_ms, 4),
+                "note": "dense B^T D B contraction only (no point physics, gather or scatter) against "
+                        "the whole sum-factorised assembly launch"}
+        for variant, name in ((0, "table_in_lds"), (1, "table_in_registers")):
+            ms, err = C.c_double(), C.c_double()
+            if capi.lib.mhx_mfma_probe(ne, 10, variant, C.byref(ms), C.byref(err)) == 0:
+                tflops = 108 * 2048.0 * ne / (ms.value * 1e-3) / 1e12
+                mfma[name] = {"ms_per_pass": round(ms.value, 4), "TFLOPs": round(tflops, 2),
+                              "mfma_frac": round(tflops / FP64_MFMA_PEAK_TFS, 4), "max_rel_err": err.value}
+
     def step(name, fn):
         t0 = time.perf_counter()
         print(f"[bench] {name} ...", file=sys.stderr, flush=True)
@@ -606,6 +623,7 @@ def main():
                      "colouring": {0: "element", 1: "block", 2: "x-segments", 3: "pencil streaming"}[colouring],
                      "segment": segment if colouring == 3 else None},
         "cpu_baseline": cpu,
+        "mfma_contraction_probe": mfma,
         "phases_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in prof.items()},
         "krylov": krylov,
         "newton": newton,

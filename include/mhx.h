@@ -563,6 +563,16 @@ void mhx_server_destroy(mhx_server* s);
  * counters and the achievable-bandwidth figure; not part of the reference. */
 int  mhx_copy_probe(int64_t n_doubles, int32_t reps, double* ms_per_copy);
 
+/* The dense B^T D B form of the Q2 hexahedron's element contraction (27 x
+ * 108 x 27 per element) on the FP64 matrix cores, v_mfma_f64_16x16x4f64,
+ * over n_elem synthetic elements: mean milliseconds per pass over them, and
+ * the largest relative error of the computed matrices (the first 64 in
+ * full, every element's entry sum) against the host.  variant 0 reads the
+ * basis table from LDS for every instruction, variant 1 holds it in
+ * registers.  The measured alternative to the sum-factorised column stage
+ * of the Q2 assembly kernels; no counterpart in the reference. */
+int  mhx_mfma_probe(int64_t n_elem, int32_t reps, int32_t variant, double* ms_per_pass, double* max_rel_err);
+
 /* Library version / build string. */
 const char* mhx_version(void);
 

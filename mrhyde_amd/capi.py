@@ -41,6 +41,7 @@ def _load() -> C.CDLL:
         "mhx_synchronize": (C.c_int, [vp]),
         "mhx_set_stream": (C.c_int, [vp, vp]),
         "mhx_copy_probe": (C.c_int, [i64, i32, C.POINTER(C.c_double)]),
+        "mhx_mfma_probe": (C.c_int, [i64, i32, i32, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
         "mhx_batch_create": (C.c_int, [C.POINTER(BatchDesc), C.POINTER(vp)]),
         "mhx_batch_solve": (C.c_int, [vp, i32]),
         "mhx_batch_results": (C.c_int, [vp, vp, vp, vp, vp]),
@@ -87,7 +88,7 @@ EXPORTED = [
     "mhx_ctx_create", "mhx_ctx_destroy", "mhx_last_error", "mhx_sizes", "mhx_crs_pattern",
     "mhx_dirichlet_mask", "mhx_set_dirichlet", "mhx_dof_coords", "mhx_assemble", "mhx_spmv", "mhx_linear_solve",
     "mhx_nonlinear_solve", "mhx_l2_error", "mhx_profile_enable", "mhx_profile_read", "mhx_profile_reset",
-    "mhx_synchronize", "mhx_set_stream", "mhx_set_option", "mhx_get_option", "mhx_copy_probe", "mhx_batch_create", "mhx_batch_solve",
+    "mhx_synchronize", "mhx_set_stream", "mhx_set_option", "mhx_get_option", "mhx_copy_probe", "mhx_mfma_probe", "mhx_batch_create", "mhx_batch_solve",
     "mhx_batch_results", "mhx_batch_info", "mhx_batch_set_wg", "mhx_batch_destroy","mhx_state_ptr", "mhx_state_upload", "mhx_state_download", "mhx_rng_fill", "mhx_version",
     "mhx_footprint", "mhx_ctx_update", "mhx_server_create", "mhx_server_submit", "mhx_server_poll",
     "mhx_server_info", "mhx_server_last_error", "mhx_server_destroy", "mhx_amg_setup", "mhx_amg_level",
