@@ -253,7 +253,7 @@ __device__ __forceinline__ double t_entry(const Geo& G, const double* ns, const 
 __global__ __launch_bounds__(kAmgBlk) void k_amg_P(Geo G, long n, const int32_t* __restrict__ rp,
                                                    const int32_t* __restrict__ ci, const double* __restrict__ A,
                                                    const double* __restrict__ dinv, const double* __restrict__ ns,
-                                                   const double* __restrict__ nrm, double omega,
+                                                   const double* __restrict__ nrm, const double* __restrict__ lam,
                                                    const int32_t* __restrict__ prp, int32_t* __restrict__ pci,
                                                    double* __restrict__ P, int* __restrict__ err) {
   constexpr int kSlots = 8 * kAmgMaxFields + 1;
@@ -281,6 +281,7 @@ __global__ __launch_bounds__(kAmgBlk) void k_amg_P(Geo G, long n, const int32_t*
     if (s < 0) atomicOr(err, 1);
     else acc[s] += ti;
   }
+  const double omega = kAmgDamping / *lam;
   const double sc = -omega * dinv[i];
   for (int k = rp[i]; k < rp[i + 1]; ++k) {
     const double tj = t_entry(G, ns, nrm, ci[k], a3, g);
@@ -662,17 +663,40 @@ __device__ __forceinline__ auto level_cols(const BoxMesh& m, const int32_t* __re
   else return CsrCols{ci};
 }
 
-// w = s * D^-1 A v  (s: scale applied to v, so v need not be normalised in place)
+// w = s * D^-1 A v  (s = *sp, on the device: the scale applied to v, so v
+// need not be normalised in place)
 template <int LANES, bool LAT>
 __global__ __launch_bounds__(256) void k_amg_dinv_spmv(BoxMesh m, long n, const int32_t* __restrict__ rp,
                                                        const int32_t* __restrict__ ci, const double* __restrict__ A,
-                                                       const double* __restrict__ dinv, double s,
+                                                       const double* __restrict__ dinv, const double* __restrict__ sp,
                                                        const double* __restrict__ v, double* __restrict__ w) {
   const long row = ((long)blockIdx.x * blockDim.x + threadIdx.x) / LANES;
   const int l = threadIdx.x % LANES;
   if (row >= n) return;
   const double acc = row_dot<LANES, double>(rp, level_cols<LAT>(m, rp, ci, row), A, v, row, l);
-  if (l == 0) w[row] = s * dinv[row] * acc;
+  if (l == 0) w[row] = *sp * dinv[row] * acc;
+}
+
+// scalar steps of the power iteration, one thread; st = (s, lambda, stopped).
+// phase 0: s = 1 / sqrt(v.v); 1: lambda = s (v.w), then s = 1 / |w| unless
+// |w| is zero or not finite, which stops the iteration (later steps leave
+// lambda alone); 2: lambda = 1 where it is not a finite positive number.
+__global__ void k_amg_power(int phase, const double* __restrict__ d0, const double* __restrict__ d1,
+                            double* __restrict__ st) {
+  if (phase == 0) {
+    st[0] = 1.0 / sqrt(*d0);
+    st[1] = 1.0;
+    st[2] = 0.0;
+  } else if (phase == 1) {
+    if (st[2] != 0.0) return;
+    st[1] = st[0] * *d0;
+    const double wn = sqrt(*d1);
+    if (!(wn > 0.0) || !isfinite(wn)) st[2] = 1.0;
+    else st[0] = 1.0 / wn;
+  } else {
+    const double lam = st[1];
+    st[1] = (isfinite(lam) && lam > 0.0) ? lam : 1.0;
+  }
 }
 
 template <int LANES, bool LAT, class T>
@@ -811,6 +835,9 @@ struct Amg {
   DevBuf<double> cinv;    // dense inverse of the coarsest operator (FP64 in both precisions)
   DevBuf<double> scratch;  // set-up temporaries
   DevBuf<int> err;
+  // power iterations on the device: two inner-product slots, then per level
+  // (s, lambda, stopped, -)
+  DevBuf<double> pw;
   const double* A0 = nullptr;  // level-0 operator of the last set-up
   bool fp32 = false;
   int smoother = kSmootherChebyshev;
@@ -869,36 +896,48 @@ static Geo upload_geo(Ctx& c, AmgLevel& L, const AxisHost* ax, int nf) {
   return G;
 }
 
-static void check_err(Ctx& c, Amg& H, const char* where) {
+// the set-up's one host synchronisation: its error bits (1, 2: an entry of
+// the operator outside the lattice stencil while forming P or R; 4: singular
+// coarsest operator) and the lambda of each of the first n_smoothed levels
+static void finish_setup(Ctx& c, Amg& H, int n_smoothed) {
   int e = 0;
+  std::vector<double> st(2 + 4 * (size_t)n_smoothed);
   MHX_HIP(hipMemcpyAsync(&e, H.err.p, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+  if (n_smoothed > 0)
+    MHX_HIP(hipMemcpyAsync(st.data(), H.pw.p, st.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
   MHX_HIP(hipStreamSynchronize(c.stream));
+  for (int l = 0; l < n_smoothed; ++l) H.lv[l]->lambda = st[2 + 4 * l + 1];
   if (e & 4) throw Error{MHX_ERR_NONFINITE, "AMG: singular coarsest operator"};
-  if (e) throw Error{MHX_ERR_ARG, std::string("AMG: matrix entries outside the lattice stencil (") + where + ")"};
+  if (e) throw Error{MHX_ERR_ARG, "AMG: matrix entries outside the lattice stencil (prolongator)"};
 }
 
-// lambda ~ rho(D^-1 A) by power iteration from a fixed start vector
-static double power_estimate(Ctx& c, AmgLevel& L, const double* A, double* v, double* w) {
+// lambda ~ rho(D^-1 A) by power iteration from a fixed start vector,
+// entirely on the stream: the scale s and the estimate stay on the device
+// (st = (s, lambda, stopped)), and the host reads every level's lambda once,
+// at the end of the set-up.  The steps are the host form's, in the same
+// floating-point operations: lambda = s (v . D^-1 A s v), s = 1 / |w|.
+static void power_estimate(Ctx& c, AmgLevel& L, const double* A, double* v, double* w, double* slots,
+                           double* st) {
   const long n = L.n;
   hipLaunchKernelGGL(k_amg_start, g1(n), dim3(256), 0, c.stream, n, v);
   MHX_HIP(hipGetLastError());
-  double s = 1.0 / std::sqrt(dot(c, v, v, n));  // v is normalised through s
-  double lam = 1.0;
+  dot_dev(c, v, v, n, slots);
+  hipLaunchKernelGGL(k_amg_power, dim3(1), dim3(1), 0, c.stream, 0, slots, slots, st);
   for (int it = 0; it < kAmgPowerIters; ++it) {
     by_lanes(L.lanes, [&](auto LN) {
       by_lat(L.lat, [&](auto LT) {
         hipLaunchKernelGGL((k_amg_dinv_spmv<decltype(LN)::value, decltype(LT)::value>), g1(n * LN), dim3(256), 0,
-                           c.stream, c.mesh, n, L.rp, L.ci, A, L.dinv.p, s, v, w);
+                           c.stream, c.mesh, n, L.rp, L.ci, A, L.dinv.p, st, v, w);
       });
     });
     MHX_HIP(hipGetLastError());
-    lam = s * dot(c, v, w, n);  // (s v) . (D^-1 A s v)
-    const double wn = std::sqrt(dot(c, w, w, n));
-    if (!(wn > 0.0) || !std::isfinite(wn)) break;
+    dot_dev(c, v, w, n, slots);
+    dot_dev(c, w, w, n, slots + 1);
+    hipLaunchKernelGGL(k_amg_power, dim3(1), dim3(1), 0, c.stream, 1, slots, slots + 1, st);
     std::swap(v, w);
-    s = 1.0 / wn;
   }
-  return (std::isfinite(lam) && lam > 0.0) ? lam : 1.0;
+  hipLaunchKernelGGL(k_amg_power, dim3(1), dim3(1), 0, c.stream, 2, slots, slots, st);
+  MHX_HIP(hipGetLastError());
 }
 
 void amg_setup(Ctx& c, const double* A, bool fp32) {
@@ -943,6 +982,7 @@ void amg_setup(Ctx& c, const double* A, bool fp32) {
     return;
   }
   H.err.alloc(1);
+  H.pw.alloc(2 + 4 * kAmgMaxLevels);
   MHX_HIP(hipMemsetAsync(H.err.p, 0, sizeof(int), c.stream));
   H.A0 = A;
   H.fp32 = fp32;
@@ -984,10 +1024,10 @@ void amg_setup(Ctx& c, const double* A, bool fp32) {
     if (nc >= L->n) break;  // no coarsening left
     Geo G = upload_geo(c, *L, ax, nf);
     const long n = L->n;
-    // lambda and omega
+    // lambda (on the device; omega = damping / lambda is formed by k_amg_P)
     H.scratch.alloc(std::max<size_t>(H.scratch.n, (size_t)2 * n));
-    L->lambda = power_estimate(c, *L, Acur, H.scratch.p, H.scratch.p + n);
-    const double omega = kAmgDamping / L->lambda;
+    double* st = H.pw.p + 2 + 4 * (nlev - 1);
+    power_estimate(c, *L, Acur, H.scratch.p, H.scratch.p + n, H.pw.p, st);
     // next level's near-null space = aggregate norms
     AmgLevel& C = next_level();
     L = H.lv[nlev - 2].get();  // the vector may have grown
@@ -1007,7 +1047,7 @@ void amg_setup(Ctx& c, const double* A, bool fp32) {
     L->P.alloc(p_nnz);
     hipLaunchKernelGGL(k_box_rowptr, g1(n + 1), dim3(256), 0, c.stream, G, 0, (int)kPext, (int)kSp, n, L->prp.p);
     hipLaunchKernelGGL(k_amg_P, g1(n, kAmgBlk), dim3(kAmgBlk), 0, c.stream, G, n, L->rp, L->ci, Acur, L->dinv.p,
-                       L->ns.p, C.ns.p, omega, L->prp.p, L->pci.p, L->P.p, H.err.p);
+                       L->ns.p, C.ns.p, st + 1, L->prp.p, L->pci.p, L->P.p, H.err.p);
     // R = P^T
     L->rrp.alloc(nc + 1);
     L->rci.alloc(r_nnz);
@@ -1027,7 +1067,6 @@ void amg_setup(Ctx& c, const double* A, bool fp32) {
     hipLaunchKernelGGL(k_amg_Ac, dim3((unsigned)nc), dim3(kAmgBlk), 0, c.stream, G, L->rrp.p, L->rci.p, L->R.p,
                        H.scratch.p, C.ns.p, C.rp_own.p, C.ci_own.p, C.A.p);
     MHX_HIP(hipGetLastError());
-    check_err(c, H, "prolongator");
     L->plane = lanes_for((double)p_nnz / (double)n);
     L->rlane = lanes_for((double)r_nnz / (double)nc);
     C.rp = C.rp_own.p;
@@ -1052,7 +1091,7 @@ void amg_setup(Ctx& c, const double* A, bool fp32) {
     H.cinv.alloc(n * n);
     hipLaunchKernelGGL(k_amg_extract_inv<double>, g1(n * n), dim3(256), 0, c.stream, n, H.scratch.p, H.cinv.p);
     MHX_HIP(hipGetLastError());
-    check_err(c, H, "coarsest");
+    finish_setup(c, H, nlev - 1);
   }
   // block-Jacobi smoother: nodal block inverses of every smoothed level
   if (H.smoother != kSmootherChebyshev) {

@@ -210,6 +210,14 @@ static double norm2_t(Ctx& c, const T* a, long n) {
   return std::sqrt(reduce_to_host<T>(c, n, a, a, 0));
 }
 
+void dot_dev(Ctx& c, const double* a, const double* b, long n, double* slot) {
+  ensure_red(c, 1);
+  const int nb = red_blocks(n);
+  hipLaunchKernelGGL(k_dot<double>, dim3(nb), dim3(256), 0, c.stream, n, a, b, c.kw.part.p, 0);
+  hipLaunchKernelGGL(k_final, dim3(1), dim3(256), 0, c.stream, c.kw.part.p, nb, 0, slot, nullptr);
+  MHX_HIP(hipGetLastError());
+}
+
 double dot(Ctx& c, const double* a, const double* b, long n) {
   ScopedPhase ph(c.prof, "Krylov::dot", c.stream);
   return reduce_to_host<double>(c, n, a, b, 0);

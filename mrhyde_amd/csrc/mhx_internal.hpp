@@ -277,6 +277,8 @@ void assemble_ns(Ctx& c, const double* u, double* res, double* jac, bool want_re
 // linear algebra (krylov.hip)
 void spmv(Ctx& c, const double* A, const double* x, double* y);
 double dot(Ctx& c, const double* a, const double* b, long n);
+// a . b reduced into *slot on the device (no host copy, no synchronisation)
+void dot_dev(Ctx& c, const double* a, const double* b, long n, double* slot);
 double norm2(Ctx& c, const double* a, long n);
 double norm_inf(Ctx& c, const double* a, long n);
 int krylov_solve(Ctx& c, const double* A, const double* b, double* x, const mhx_krylov_opts& o,
