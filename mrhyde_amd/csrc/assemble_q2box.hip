@@ -625,14 +625,23 @@ __global__ __launch_bounds__(64) void k_q2box(AsmArgs args) {
         }
         const int X = 2 * ex + i1;
         const int lox = (X & 1) ? X - 1 : (X - 2 < 0 ? 0 : X - 2);
+        if (__any(fixed)) {
 #pragma unroll
-        for (int k = 0; k < (r < 6 ? 3 : 5); ++k) {
-          const bool first = (firstm >> k) & 1;
-          if (k >= n || (fixed && !first)) continue;
-          // strong Dirichlet row: e_d, written by its first toucher
-          const double v = fixed ? ((i2 == j2 && i3 == j3 && lox + ox + k == X) ? 1.0 : 0.0)
-                                 : (first ? val[k] : old[r][k] + val[k]);
-          ga_jac[off + k] = v;
+          for (int k = 0; k < (r < 6 ? 3 : 5); ++k) {
+            const bool first = (firstm >> k) & 1;
+            if (k >= n || (fixed && !first)) continue;
+            // strong Dirichlet row: e_d, written by its first toucher
+            const double v = fixed ? ((i2 == j2 && i3 == j3 && lox + ox + k == X) ? 1.0 : 0.0)
+                                   : (first ? val[k] : old[r][k] + val[k]);
+            ga_jac[off + k] = v;
+          }
+        } else {
+          // no strong-Dirichlet row in the round (the common case): a first
+          // touch read nothing (old = +0) and val is never -0 (sums started
+          // from +0), so old + val is val exactly
+#pragma unroll
+          for (int k = 0; k < (r < 6 ? 3 : 5); ++k)
+            if (k < n) ga_jac[off + k] = old[r][k] + val[k];
         }
       }
   }
