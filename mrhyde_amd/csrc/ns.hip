@@ -390,17 +390,29 @@ __device__ __forceinline__ void ns_seg_scatter(const AsmArgs& a, const NsSeg& u,
         // the next round's read-backs go out before this round's stores
         // (disjoint rows; loads issued ahead of stores do not wait for them)
         if (fr + 1 < NF) ns_seg_readback<NIT, NF>(a, s_rps, it_r, it_off, it_f, fr + 1, addr_n, old_n, fix_n);
+        bool anyfix = false;
 #pragma unroll
-        for (int k = 0; k < NIT; ++k) {
-          const unsigned f = it_f[k];
-          if (!(f & 1u)) continue;
-          const bool first = f & 2u;
-          if (fixd[k]) {
-            if (first) a.jac[addr[k]] = ((f & 4u) && (int)(f >> 4) == fr) ? 1.0 : 0.0;
-          } else {
-            const double v = jb[t + k * NT];
-            a.jac[addr[k]] = first ? v : old[k] + v;
+        for (int k = 0; k < NIT; ++k) anyfix = anyfix || ((it_f[k] & 1u) && fixd[k]);
+        if (__any(anyfix)) {
+#pragma unroll
+          for (int k = 0; k < NIT; ++k) {
+            const unsigned f = it_f[k];
+            if (!(f & 1u)) continue;
+            const bool first = f & 2u;
+            if (fixd[k]) {
+              if (first) a.jac[addr[k]] = ((f & 4u) && (int)(f >> 4) == fr) ? 1.0 : 0.0;
+            } else {
+              const double v = jb[t + k * NT];
+              a.jac[addr[k]] = first ? v : old[k] + v;
+            }
           }
+        } else {
+          // no strong-Dirichlet row among the wavefront's entries (the
+          // common case): a first touch read nothing (old = +0), so old + v
+          // is v (up to the sign of a zero)
+#pragma unroll
+          for (int k = 0; k < NIT; ++k)
+            if (it_f[k] & 1u) a.jac[addr[k]] = old[k] + jb[t + k * NT];
         }
 #pragma unroll
         for (int k = 0; k < NIT; ++k) {
