@@ -426,8 +426,13 @@ __device__ __forceinline__ void ns_seg_scatter(const AsmArgs& a, const NsSeg& u,
   }
 }
 
-template <int DIM, bool JAC, bool SEG>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_ns(AsmArgs a, NsCoef nc) {
+// STAB: SUPG and PSPG both on, known at compile time (the stabilisation
+// flags of the local copy become constants, so neither term is selected per
+// lane at run time)
+template <int DIM, bool JAC, bool SEG, bool STAB>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_ns(AsmArgs a, NsCoef nc_arg) {
+  NsCoef nc = nc_arg;
+  if constexpr (STAB) nc.supg = 1, nc.pspg = 1;
   using C = NsCfg<DIM>;
   constexpr int NV = C::NV, NQ = C::NQ, NF = C::NF, NE = C::NE, EPW = C::EPW, NQS = C::NQS;
   __shared__ double s_u[EPW][NE];
@@ -813,13 +818,18 @@ static void launch_ns(Ctx& c, AsmArgs& a, const NsCoef& nc, bool jac, bool seg) 
     const long blocks = seg ? cnt : (cnt + C::EPW - 1) / C::EPW;
     ScopedPhase ph(c.prof, jac ? "k_ns<jacres>" : "k_ns<res>", c.stream);
     const dim3 grid((unsigned)blocks), block(C::NT);
-    if (seg) {
-      if (jac) hipLaunchKernelGGL((k_ns<DIM, true, true>), grid, block, 0, c.stream, a, nc);
-      else hipLaunchKernelGGL((k_ns<DIM, false, true>), grid, block, 0, c.stream, a, nc);
-    } else {
-      if (jac) hipLaunchKernelGGL((k_ns<DIM, true, false>), grid, block, 0, c.stream, a, nc);
-      else hipLaunchKernelGGL((k_ns<DIM, false, false>), grid, block, 0, c.stream, a, nc);
-    }
+    auto go = [&](auto ST) {
+      constexpr bool S = decltype(ST)::value;
+      if (seg) {
+        if (jac) hipLaunchKernelGGL((k_ns<DIM, true, true, S>), grid, block, 0, c.stream, a, nc);
+        else hipLaunchKernelGGL((k_ns<DIM, false, true, S>), grid, block, 0, c.stream, a, nc);
+      } else {
+        if (jac) hipLaunchKernelGGL((k_ns<DIM, true, false, S>), grid, block, 0, c.stream, a, nc);
+        else hipLaunchKernelGGL((k_ns<DIM, false, false, S>), grid, block, 0, c.stream, a, nc);
+      }
+    };
+    if (nc.supg && nc.pspg) go(std::true_type{});
+    else go(std::false_type{});
     MHX_HIP(hipGetLastError());
   }
 }
