@@ -473,9 +473,10 @@ int  mhx_set_stream(mhx_ctx* ctx, void* hip_stream);
 int  mhx_rng_fill(uint64_t seed, int64_t n, int32_t kind, double a, double b, double* out);
 
 /* ---- batched multiscale subgrid solves ----------------------------------
- * Many independent thermal subproblems, each the unit cube cut into n^3 Q1
- * hexahedra (2 <= n <= 16, 2x2x2 Gauss), element conductivities k, strong
- * Dirichlet data on every boundary node, no source; each is assembled and
+ * Many independent thermal subproblems, each a box (the unit cube unless
+ * extent is given) cut into n^3 Q1 hexahedra (2 <= n <= 16, 2x2x2 Gauss),
+ * element conductivities k, optional element sources, strong Dirichlet data
+ * on every boundary node; each is assembled and
  * solved with Jacobi-PCG to ||b - A x|| / ||b|| <= tol inside ONE persistent
  * launch (optionally replayed from a captured hipGraph).  FP32_REFINED runs
  * the inner PCG in FP32 inside FP64 iterative refinement (FP64 residuals).
@@ -498,6 +499,8 @@ typedef struct mhx_batch_desc {
   int32_t max_refine;      /* default 20 */
   int32_t wg_size;         /* 512 or 1024; 0 = autotune on this batch */
   int32_t use_graph;       /* 1 = capture the launch once and replay it */
+  const double* extent;    /* [n_inst][3] box side lengths (Lx, Ly, Lz); NULL = unit cubes (host) */
+  const double* src;       /* concatenated n^3 element sources f_e; NULL = none (host) */
   int32_t reserved[4];
 } mhx_batch_desc;
 int  mhx_batch_create(const mhx_batch_desc* desc, mhx_batch** out);

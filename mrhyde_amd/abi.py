@@ -107,7 +107,8 @@ class BatchDesc(C.Structure):
     _fields_ = [
         ("n_inst", C.c_int32), ("n_axis", C.POINTER(C.c_int32)), ("k", _dp), ("g", _dp),
         ("tol", C.c_double), ("max_iters", C.c_int32), ("precision", C.c_int32), ("inner_tol", C.c_double),
-        ("max_refine", C.c_int32), ("wg_size", C.c_int32), ("use_graph", C.c_int32), ("reserved", C.c_int32 * 4),
+        ("max_refine", C.c_int32), ("wg_size", C.c_int32), ("use_graph", C.c_int32), ("extent", _dp), ("src", _dp),
+        ("reserved", C.c_int32 * 4),
     ]
 
 

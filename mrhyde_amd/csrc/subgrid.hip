@@ -10,10 +10,12 @@
 //                                                    subgridDtN_solver.cpp:903-1043, :681-902
 // with the PCG of SolverManager::PCG (solverManager_solvers.hpp:812-905).
 //
-// Instance: [0,1]^3 cut into n^3 Q1 hexahedra (n per instance), element
-// conductivity k_e, Dirichlet data g on every boundary node, no source.  The
-// system is J du = -F(u0) with u0 = g on the boundary (reference strong-DBC
-// convention), solved on the interior nodes; the result is u = u0 + du.
+// Instance: a box [0,Lx]x[0,Ly]x[0,Lz] (the unit cube by default) cut into
+// n^3 Q1 hexahedra (n per instance), element conductivity k_e, optional
+// element source f_e, Dirichlet data g on every boundary node.  The system is
+// J du = -F(u0) with u0 = g on the boundary (reference strong-DBC convention)
+// and F(u) = A u - b, b_i = sum over the elements e of node i of f_e |e| / 8,
+// solved on the interior nodes; the result is u = u0 + du.
 //
 // Mapping: one workgroup owns one instance at a time and pulls the next from
 // a global counter (largest instances first) until the queue is empty, so
@@ -21,9 +23,9 @@
 // direction p and the element conductivities in LDS, each thread's share of
 // x, r and the Jacobi scaling in registers.  The operator is applied node by
 // node from the 27 neighbouring p values and the 8 surrounding element
-// conductivities with the closed-form unit-cube Q1 stiffness (exact for the
-// 2x2x2 Gauss rule the reference uses), i.e. the assembled 27-point row is
-// formed on the fly instead of being stored.
+// conductivities with the closed-form Q1 stiffness of the instance's box
+// element (exact for the 2x2x2 Gauss rule the reference uses), i.e. the
+// assembled 27-point row is formed on the fly instead of being stored.
 
 #include <algorithm>
 #include <cmath>
@@ -34,7 +36,22 @@
 
 namespace mhx {
 
-// ---- unit-cube trilinear stiffness K = S(x)M(x)M + M(x)S(x)M + M(x)M(x)S -----
+// ---- box trilinear stiffness -------------------------------------------------
+// On a box element of sides (hx, hy, hz) the Q1 stiffness is
+//   K = cx S(x)M(x)M + cy M(x)S(x)M + cz M(x)M(x)S,
+//   cx = hy hz / hx, cy = hx hz / hy, cz = hx hy / hz,
+// with the 1-D S = [1 -1; -1 1] and M = [1/3 1/6; 1/6 1/3].  An entry depends
+// only on which of the two nodes' coordinates differ: kv[dx + 2 dy + 4 dz].
+inline void box_stencil(double hx, double hy, double hz, double kv[8]) {
+  const double cx = hy * hz / hx, cy = hx * hz / hy, cz = hx * hy / hz;
+  const double S[2] = {1.0, -1.0}, M[2] = {1.0 / 3.0, 1.0 / 6.0};
+  for (int p = 0; p < 8; ++p) {
+    const int dx = p & 1, dy = (p >> 1) & 1, dz = (p >> 2) & 1;
+    kv[p] = cx * S[dx] * M[dy] * M[dz] + cy * M[dx] * S[dy] * M[dz] + cz * M[dx] * M[dy] * S[dz];
+  }
+}
+
+// the unit-cube table (hx = hy = hz = 1): K_h = h K_unit for a cube of side h
 struct K8 {
   double v[8][8];
 };
@@ -62,6 +79,8 @@ struct SgInst {
   int k_off;   // offset of the instance's element coefficients
   int x_off;   // offset of the instance's nodal arrays
   int pad;
+  double kv[8];  // element stiffness entries by coordinate-difference pattern
+  double vol8;   // element volume / 8: the share of a node in a constant source
 };
 
 struct SgArgs {
@@ -69,6 +88,7 @@ struct SgArgs {
   const int* order;      // queue position -> instance (largest first)
   int n_inst;
   const double* k;
+  const double* src;     // element sources (nullptr: none)
   const double* g;
   double* x;
   int* iters;
@@ -84,11 +104,14 @@ struct SgArgs {
 
 __global__ void k_sg_reset(unsigned int* q) { *q = 0u; }
 
-// q = A p at node (ix,iy,iz) (interior), p and k in LDS; h = 1/n scaling
-// (the element stiffness of an h-cube is h * K_unit).
-template <class T>
-__device__ __forceinline__ T sg_apply(const T* __restrict__ p, const T* __restrict__ kk, int n, T h, int ix, int iy,
-                                      int iz) {
+// q = A p at node (ix,iy,iz) (interior), p and k in LDS.  BOX (the general
+// instance: a box, possibly with sources): kv holds the box element's
+// stiffness entries by coordinate-difference pattern; else the instance is
+// the unit cube without sources and its element stiffness h K_unit (h = 1/n)
+// comes from the compile-time table.
+template <class T, bool BOX>
+__device__ __forceinline__ T sg_apply(const T* __restrict__ p, const T* __restrict__ kk, int n, const T (&kv)[8],
+                                      int ix, int iy, int iz) {
   const int n1 = n + 1;
   T P[3][3][3];
 #pragma unroll
@@ -107,23 +130,39 @@ __device__ __forceinline__ T sg_apply(const T* __restrict__ p, const T* __restri
 #pragma unroll
     for (int b = 0; b < 8; ++b) {
       const int bx = b & 1, by = (b >> 1) & 1, bz = (b >> 2) & 1;
-      s += (T)kK8.v[a][b] * P[oz + bz][oy + by][ox + bx];
+      const int ax = a & 1, ay = (a >> 1) & 1, az = (a >> 2) & 1;
+      const T kab = BOX ? kv[(ax ^ bx) | ((ay ^ by) << 1) | ((az ^ bz) << 2)] : (T)kK8.v[a][b];
+      s += kab * P[oz + bz][oy + by][ox + bx];
     }
     acc += ke * s;
   }
-  return acc * h;
+  if constexpr (BOX) return acc;
+  else return acc * ((T)1 / (T)n);
 }
 
-template <class T>
-__device__ __forceinline__ T sg_diag(const T* __restrict__ kk, int n, int ix, int iy, int iz) {
+// b_i / vol8: the sum of the element sources around node (ix,iy,iz)
+__device__ __forceinline__ double sg_src(const double* __restrict__ f, int n, int ix, int iy, int iz) {
+  double s = 0.0;
+#pragma unroll
+  for (int o = 0; o < 8; ++o) {
+    const int ox = o & 1, oy = (o >> 1) & 1, oz = (o >> 2) & 1;
+    s += f[(ix - 1 + ox) + n * ((iy - 1 + oy) + n * (iz - 1 + oz))];
+  }
+  return s;
+}
+
+template <class T, bool BOX>
+__device__ __forceinline__ T sg_diag(const T* __restrict__ kk, int n, T kv0, int ix, int iy, int iz) {
   T d = 0;
 #pragma unroll
   for (int o = 0; o < 8; ++o) {
     const int ox = o & 1, oy = (o >> 1) & 1, oz = (o >> 2) & 1;
     const int a = (1 - ox) + 2 * (1 - oy) + 4 * (1 - oz);
-    d += kk[(ix - 1 + ox) + n * ((iy - 1 + oy) + n * (iz - 1 + oz))] * (T)kK8.v[a][a];
+    const T ke = kk[(ix - 1 + ox) + n * ((iy - 1 + oy) + n * (iz - 1 + oz))];
+    d += BOX ? ke : ke * (T)kK8.v[a][a];
   }
-  return d / (T)n;  // once per node at set-up
+  if constexpr (BOX) return d * kv0;  // once per node at set-up
+  else return d / (T)n;
 }
 
 // Sum of two values over the workgroup, returned to every thread.
@@ -153,11 +192,11 @@ __device__ __forceinline__ void wg_sum2(T& a, T& b, T* red) {
 // Jacobi-PCG on the interior nodes (SolverManager::PCG algorithm):
 //   x += CG solution of A x = r_in (r_in given in the thread's slots),
 // stopping when ||r|| / ||r_in|| <= tol.  Returns iterations; rr_out = ||r||^2.
-template <int WG, int S, class T>
-__device__ __forceinline__ int sg_pcg(int n, const T* kk, T* p_lds, T* red, T (&x)[S], T (&r)[S], const T (&dinv)[S], bool (&in)[S],
-                      int (&node)[S], double tol, int max_iters, T& rr_out) {
+template <int WG, int S, class T, bool BOX>
+__device__ __forceinline__ int sg_pcg(int n, const T* kk, const T (&kv)[8], T* p_lds, T* red, T (&x)[S], T (&r)[S],
+                                      const T (&dinv)[S], bool (&in)[S], int (&node)[S], double tol, int max_iters,
+                                      T& rr_out) {
   const int n1 = n + 1, nn = n1 * n1 * n1;
-  const T h = (T)1 / (T)n;
   T pp[S];
   T rr = 0, rz = 0;
 #pragma unroll
@@ -191,7 +230,7 @@ __device__ __forceinline__ int sg_pcg(int n, const T* kk, T* p_lds, T* red, T (&
         q[s] = 0;
         if (in[s]) {
           const int v = node[s];
-          q[s] = sg_apply<T>(p_lds, kk, n, h, v % n1, (v / n1) % n1, v / (n1 * n1));
+          q[s] = sg_apply<T, BOX>(p_lds, kk, n, kv, v % n1, (v / n1) % n1, v / (n1 * n1));
         }
         pq += pp[s] * q[s];
       }
@@ -223,7 +262,7 @@ __device__ __forceinline__ int sg_pcg(int n, const T* kk, T* p_lds, T* red, T (&
   return it;
 }
 
-template <int WG, int S, bool MIXED>
+template <int WG, int S, bool MIXED, bool BOX>
 __global__ __launch_bounds__(WG) void k_subgrid(SgArgs a) {
   __shared__ double p64[kSgMaxNodes];
   __shared__ double k64[kSgMaxElems];
@@ -236,6 +275,8 @@ __global__ __launch_bounds__(WG) void k_subgrid(SgArgs a) {
   for (;;) {
     if (threadIdx.x == 0) s_next = (int)atomicAdd(a.queue, 1u);
     __syncthreads();
+    // (wave-uniform: the instance's description is then read with scalar
+    // loads and its stencil entries stay in scalar registers)
     const int qpos = s_next;
     __syncthreads();
     if (qpos >= a.n_inst) break;
@@ -243,7 +284,11 @@ __global__ __launch_bounds__(WG) void k_subgrid(SgArgs a) {
     const SgInst in_ = a.inst[id];
     const int n = in_.n, n1 = n + 1, nn = n1 * n1 * n1, ne = n * n * n;
     const double* kg = a.k + in_.k_off;
+    const double* fg = a.src ? a.src + in_.k_off : nullptr;
     const double* gg = a.g + in_.x_off;
+    double kv[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) kv[i] = in_.kv[i];
 
     for (int e = threadIdx.x; e < ne; e += WG) k64[e] = kg[e];
     int node[S];
@@ -258,7 +303,7 @@ __global__ __launch_bounds__(WG) void k_subgrid(SgArgs a) {
       if (v < nn) p64[v] = bnd ? gg[v] : 0.0;  // u0: the Dirichlet lift
     }
     __syncthreads();
-    // b = -F(u0) = -A u0 on interior rows (assembled residual with the Dirichlet lift)
+    // b - A u0 = -F(u0) on interior rows (assembled residual with the Dirichlet lift)
     double r[S], x[S], dinv[S];
 #pragma unroll
     for (int s = 0; s < S; ++s) {
@@ -269,8 +314,9 @@ __global__ __launch_bounds__(WG) void k_subgrid(SgArgs a) {
       if (in[s]) {
         const int v = node[s];
         const int ix = v % n1, iy = (v / n1) % n1, iz = v / (n1 * n1);
-        r[s] = -sg_apply<double>(p64, k64, n, 1.0 / n, ix, iy, iz);
-        dinv[s] = 1.0 / sg_diag<double>(k64, n, ix, iy, iz);
+        const double b = (BOX && fg) ? in_.vol8 * sg_src(fg, n, ix, iy, iz) : 0.0;
+        r[s] = b - sg_apply<double, BOX>(p64, k64, n, kv, ix, iy, iz);
+        dinv[s] = 1.0 / sg_diag<double, BOX>(k64, n, kv[0], ix, iy, iz);
       }
     }
     __syncthreads();
@@ -280,7 +326,7 @@ __global__ __launch_bounds__(WG) void k_subgrid(SgArgs a) {
     for (int s = 0; s < S; ++s) bb += r[s] * r[s];
     wg_sum2<WG>(bb, dummy, red);
     if constexpr (!MIXED) {
-      its = sg_pcg<WG, S, double>(n, k64, p64, red, x, r, dinv, in, node, a.tol, a.max_iters, rr);
+      its = sg_pcg<WG, S, double, BOX>(n, k64, kv, p64, red, x, r, dinv, in, node, a.tol, a.max_iters, rr);
     } else {
       // FP32 inner PCG inside FP64 iterative refinement: the residual, the
       // accumulated correction and the convergence test stay FP64.
@@ -299,10 +345,12 @@ __global__ __launch_bounds__(WG) void k_subgrid(SgArgs a) {
           df[s] = (float)dinv[s];
         }
         __syncthreads();
-        float rrf = 0.f;
-        its += sg_pcg<WG, S, float>(n, k32, p32, red32, xf, rf, df, in, node, a.inner_tol, a.max_iters - its, rrf);
+        float rrf = 0.f, kvf[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) kvf[i] = (float)kv[i];
+        its += sg_pcg<WG, S, float, BOX>(n, k32, kvf, p32, red32, xf, rf, df, in, node, a.inner_tol, a.max_iters - its, rrf);
         ++refs;
-        // x += scale * xf ; r = b - A x = -A (u0 + x) in FP64
+        // x += scale * xf ; r = b - A (u0 + x) in FP64
         __syncthreads();
 #pragma unroll
         for (int s = 0; s < S; ++s) {
@@ -318,7 +366,8 @@ __global__ __launch_bounds__(WG) void k_subgrid(SgArgs a) {
           if (in[s]) {
             const int v = node[s];
             const int ix = v % n1, iy = (v / n1) % n1, iz = v / (n1 * n1);
-            r[s] = -sg_apply<double>(p64, k64, n, 1.0 / n, ix, iy, iz);
+            const double b = (BOX && fg) ? in_.vol8 * sg_src(fg, n, ix, iy, iz) : 0.0;
+            r[s] = b - sg_apply<double, BOX>(p64, k64, n, kv, ix, iy, iz);
           }
           rr2 += r[s] * r[s];
         }
@@ -355,12 +404,13 @@ struct Batch {
   std::vector<int> h_order;
   DevBuf<SgInst> inst;
   DevBuf<int> order, iters, refs;
-  DevBuf<double> k, g, x, relres;
+  DevBuf<double> k, src, g, x, relres;
   DevBuf<unsigned int> queue;
   hipStream_t stream = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   int graph_wg = 0;
+  bool box = false;  // instances with their own side lengths or sources (else unit cubes, no source)
   int wg = 0;
   int grid[2] = {0, 0};  // per WG variant (512, 1024)
   mhx_batch_desc desc{};
@@ -379,6 +429,7 @@ static SgArgs make_args(Batch& b) {
   a.order = b.order.p;
   a.n_inst = b.n_inst;
   a.k = b.k.p;
+  a.src = b.src.p;
   a.g = b.g.p;
   a.x = b.x.p;
   a.iters = b.iters.p;
@@ -397,17 +448,22 @@ static void enqueue(Batch& b, int wg) {
   SgArgs a = make_args(b);
   hipLaunchKernelGGL(k_sg_reset, dim3(1), dim3(1), 0, b.stream, b.queue.p);
   constexpr int S1024 = (kSgMaxNodes + 1023) / 1024, S512 = (kSgMaxNodes + 511) / 512;
-  if (wg == 1024) {
-    if (a.fp32)
-      hipLaunchKernelGGL((k_subgrid<1024, S1024, true>), dim3(b.grid[1]), dim3(1024), 0, b.stream, a);
-    else
-      hipLaunchKernelGGL((k_subgrid<1024, S1024, false>), dim3(b.grid[1]), dim3(1024), 0, b.stream, a);
-  } else {
-    if (a.fp32)
-      hipLaunchKernelGGL((k_subgrid<512, S512, true>), dim3(b.grid[0]), dim3(512), 0, b.stream, a);
-    else
-      hipLaunchKernelGGL((k_subgrid<512, S512, false>), dim3(b.grid[0]), dim3(512), 0, b.stream, a);
-  }
+  auto go = [&](auto BX) {
+    constexpr bool X = decltype(BX)::value;
+    if (wg == 1024) {
+      if (a.fp32)
+        hipLaunchKernelGGL((k_subgrid<1024, S1024, true, X>), dim3(b.grid[1]), dim3(1024), 0, b.stream, a);
+      else
+        hipLaunchKernelGGL((k_subgrid<1024, S1024, false, X>), dim3(b.grid[1]), dim3(1024), 0, b.stream, a);
+    } else {
+      if (a.fp32)
+        hipLaunchKernelGGL((k_subgrid<512, S512, true, X>), dim3(b.grid[0]), dim3(512), 0, b.stream, a);
+      else
+        hipLaunchKernelGGL((k_subgrid<512, S512, false, X>), dim3(b.grid[0]), dim3(512), 0, b.stream, a);
+    }
+  };
+  if (b.box) go(std::true_type{});
+  else go(std::false_type{});
   MHX_HIP(hipGetLastError());
 }
 
@@ -451,9 +507,9 @@ static void setup_grid(Batch& b) {
   MHX_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
   int occ512 = 0, occ1024 = 0;
   MHX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-      &occ512, (const void*)k_subgrid<512, (kSgMaxNodes + 511) / 512, false>, 512, 0));
+      &occ512, (const void*)k_subgrid<512, (kSgMaxNodes + 511) / 512, false, false>, 512, 0));
   MHX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-      &occ1024, (const void*)k_subgrid<1024, (kSgMaxNodes + 1023) / 1024, false>, 1024, 0));
+      &occ1024, (const void*)k_subgrid<1024, (kSgMaxNodes + 1023) / 1024, false, false>, 1024, 0));
   MHX_REQUIRE(occ512 > 0 && occ1024 > 0, MHX_ERR_DEVICE, "subgrid kernel does not fit on a CU");
   b.grid[0] = std::max(1, std::min(b.n_inst, cus * occ512));
   b.grid[1] = std::max(1, std::min(b.n_inst, cus * occ1024));
@@ -481,12 +537,24 @@ int mhx_batch_create(const mhx_batch_desc* d, mhx_batch** out) {
     MHX_REQUIRE(b->desc.precision == MHX_BATCH_FP64 || b->desc.precision == MHX_BATCH_FP32_REFINED, MHX_ERR_ARG,
                 "unknown batch precision");
     b->n_inst = d->n_inst;
+    b->box = d->extent != nullptr || d->src != nullptr;
     b->h_inst.resize(b->n_inst);
     long ko = 0, xo = 0;
     for (int i = 0; i < b->n_inst; ++i) {
       const int n = d->n_axis[i];
       MHX_REQUIRE(n >= 2 && n <= kSgMaxN, MHX_ERR_UNSUPPORTED, "subgrid instances need 2..16 elements per axis");
-      b->h_inst[i] = SgInst{n, (int)ko, (int)xo, 0};
+      double L[3] = {1.0, 1.0, 1.0};
+      if (d->extent)
+        for (int c = 0; c < 3; ++c) {
+          L[c] = d->extent[3 * (long)i + c];
+          MHX_REQUIRE(std::isfinite(L[c]) && L[c] > 0.0, MHX_ERR_ARG, "subgrid box extents must be positive");
+        }
+      SgInst& si = b->h_inst[i];
+      si = SgInst{};
+      si.n = n, si.k_off = (int)ko, si.x_off = (int)xo;
+      const double hx = L[0] / n, hy = L[1] / n, hz = L[2] / n;
+      box_stencil(hx, hy, hz, si.kv);
+      si.vol8 = hx * hy * hz / 8.0;
       ko += (long)n * n * n;
       xo += (long)(n + 1) * (n + 1) * (n + 1);
     }
@@ -504,12 +572,14 @@ int mhx_batch_create(const mhx_batch_desc* d, mhx_batch** out) {
     b->refs.alloc(b->n_inst);
     b->relres.alloc(b->n_inst);
     b->k.alloc(ko);
+    if (d->src) b->src.alloc(ko);
     b->g.alloc(xo);
     b->x.alloc(xo);
     b->queue.alloc(1);
     MHX_HIP(hipMemcpyAsync(b->inst.p, b->h_inst.data(), b->n_inst * sizeof(SgInst), hipMemcpyHostToDevice, b->stream));
     MHX_HIP(hipMemcpyAsync(b->order.p, b->h_order.data(), b->n_inst * sizeof(int), hipMemcpyHostToDevice, b->stream));
     MHX_HIP(hipMemcpyAsync(b->k.p, d->k, ko * sizeof(double), hipMemcpyHostToDevice, b->stream));
+    if (d->src) MHX_HIP(hipMemcpyAsync(b->src.p, d->src, ko * sizeof(double), hipMemcpyHostToDevice, b->stream));
     MHX_HIP(hipMemcpyAsync(b->g.p, d->g, xo * sizeof(double), hipMemcpyHostToDevice, b->stream));
     MHX_HIP(hipStreamSynchronize(b->stream));
     setup_grid(*b);

@@ -469,7 +469,10 @@ class SubgridBatch:
     (src/subgrid/subgridDtN_solver.cpp:903-1043)."""
 
     def __init__(self, n_axis, k, g, tol: float = 1e-10, precision: str = "fp64", wg_size: int = 0,
-                 use_graph: bool = True, max_iters: int = 10000, inner_tol: float = 1e-5, max_refine: int = 20):
+                 use_graph: bool = True, max_iters: int = 10000, inner_tol: float = 1e-5, max_refine: int = 20,
+                 extent=None, src=None):
+        """extent: [n_inst, 3] box side lengths (default unit cubes); src:
+        concatenated element sources like k (default none)."""
         self.n_axis = np.ascontiguousarray(n_axis, dtype=np.int32)
         self._k = np.ascontiguousarray(k, dtype=np.float64)
         self._g = np.ascontiguousarray(g, dtype=np.float64)
@@ -478,6 +481,14 @@ class SubgridBatch:
         d.n_axis = self.n_axis.ctypes.data_as(C.POINTER(C.c_int32))
         d.k = self._k.ctypes.data_as(C.POINTER(C.c_double))
         d.g = self._g.ctypes.data_as(C.POINTER(C.c_double))
+        if extent is not None:
+            self._ext = np.ascontiguousarray(extent, dtype=np.float64).reshape(len(self.n_axis), 3)
+            d.extent = self._ext.ctypes.data_as(C.POINTER(C.c_double))
+        if src is not None:
+            self._src = np.ascontiguousarray(src, dtype=np.float64)
+            if self._src.shape != self._k.shape:
+                raise ValueError("src needs one value per element, like k")
+            d.src = self._src.ctypes.data_as(C.POINTER(C.c_double))
         d.tol, d.max_iters = float(tol), int(max_iters)
         d.precision = capi.MHX_BATCH_FP32_REFINED if precision == "fp32-refined" else capi.MHX_BATCH_FP64
         d.inner_tol, d.max_refine = float(inner_tol), int(max_refine)

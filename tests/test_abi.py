@@ -85,3 +85,17 @@ def test_rng_matches_restatement(kind, a, b):
         assert out.min() >= a and out.max() < b and abs(out.mean()) < 0.03
     if kind == 2:
         assert abs(out.mean()) < 0.03 and abs(out.std() - 1.0) < 0.03
+
+
+def test_batch_rejects_bad_boxes():
+    """A non-positive box side is refused before any device work; a source
+    array of the wrong length is refused by the binding."""
+    import numpy as np
+
+    from mrhyde_amd import MhxError, SubgridBatch
+
+    k, g = np.ones(8), np.zeros(27)
+    with pytest.raises(MhxError):
+        SubgridBatch(np.array([2], dtype=np.int32), k, g, extent=[[1.0, 0.0, 1.0]])
+    with pytest.raises(ValueError):
+        SubgridBatch(np.array([2], dtype=np.int32), k, g, src=np.ones(7))

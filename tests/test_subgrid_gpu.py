@@ -112,3 +112,39 @@ def test_full_batch_properties():
         bnd[1:-1, 1:-1, 1:-1] = False
         np.testing.assert_array_equal(xi[bnd], gi[bnd])
         assert xi.min() >= gi[bnd].min() - 1e-9 and xi.max() <= gi[bnd].max() + 1e-9
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32-refined"])
+def test_box_and_source_batch_matches_oracle(precision):
+    """Instances on boxes of their own side lengths with element sources:
+    the box stiffness (cx S(x)M(x)M + ...) and the source load f_e |e| / 8
+    against the oracle's quadrature assembly of the same box and source."""
+    from oracle import Oracle
+    from mrhyde_amd import SubgridBatch
+
+    rng = np.random.default_rng(21)
+    n_axis = np.array([2, 3, 5, 8, 4, 6, 16, 3], dtype=np.int32)
+    ext = rng.uniform(0.3, 2.5, (len(n_axis), 3))
+    ne = n_axis.astype(np.int64) ** 3
+    k = np.exp(rng.standard_normal(int(ne.sum())))
+    f = rng.uniform(-5.0, 5.0, int(ne.sum()))
+    g = rng.uniform(-1, 1, int(((n_axis.astype(np.int64) + 1) ** 3).sum()))
+    b = SubgridBatch(n_axis, k, g, tol=1e-10, precision=precision, use_graph=False, extent=ext, src=f)
+    b.solve()
+    x, it, rel, _ = b.results()
+    ko = np.concatenate([[0], np.cumsum(ne)])
+    for i, n in enumerate(n_axis):
+        print(f"[subgrid box] instance {i} n={n} L={ext[i]} relres={rel[i]:.2e}", flush=True)
+        xi, gi = b.instance(x, i), b.instance(g, i)
+        o = Oracle(Problem(dim=3, order=1, quadrature=2, n=(int(n),) * 3, hi=tuple(ext[i]), k0=k[ko[i]:ko[i + 1]],
+                           src_elem=f[ko[i]:ko[i + 1]], dbc_values=np.asarray(gi)))
+        u0 = o.setDirichlet(np.zeros(o.n_dof))
+        res, jac = o.assembleJacRes(u0)
+        opts = KrylovOpts()
+        opts.solver, opts.tol, opts.max_iters = MHX_KRYLOV_CG, 1e-13, 20000
+        du, _ = o.linearSolver(jac, res, opts=opts)
+        u_ref = u0 + du
+        rp, ci = o.crs_pattern()
+        r_true = res - spmv_np(rp, ci, jac, xi - u0)
+        assert rel[i] <= 1e-10 and np.linalg.norm(r_true) <= 2e-10 * np.linalg.norm(res), (i, rel[i])
+        np.testing.assert_allclose(xi, u_ref, rtol=0, atol=1e-8 * np.abs(u_ref).max())
