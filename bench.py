@@ -40,7 +40,7 @@ LDS_PEAK_TBS = 256 * 128 * 2.4e9 / 1e12  # 256 CUs x 128 B/clk x 2.4 GHz
 # of 12 elements (profiles/k_q2box_q2hex_96cubed.md).  Only valid for those
 # workloads.
 PMC_TRAFFIC = {(96, 0): {"fetch_kib": 529028.5, "write_kib": 1032647.0, "fetch_scale": 2048, "write_scale": 1024},
-               (96, 3): {"fetch_kib": 327241.4, "write_kib": 722878.3, "fetch_scale": 2048, "write_scale": 1024,
+               (96, 3): {"fetch_kib": 326291.6, "write_kib": 720445.1, "fetch_scale": 2048, "write_scale": 1024,
                          "segment": 12}}
 
 
