@@ -184,8 +184,21 @@ extern "C" int mhx_mfma_probe(int64_t n_elem, int32_t reps, int32_t variant, dou
     sums.alloc((size_t)n_elem);
     full.alloc((size_t)n_full * 4 * 64 * 4);
     lay.alloc(256);
-    hipStream_t s;
-    MHX_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    // stream and events released on every exit path (the MHX_HIP throws included)
+    struct Res {
+      hipStream_t s = nullptr;
+      hipEvent_t a = nullptr, b = nullptr;
+      ~Res() {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+        if (s) (void)hipStreamDestroy(s);
+      }
+    } res;
+    MHX_HIP(hipStreamCreateWithFlags(&res.s, hipStreamNonBlocking));
+    MHX_HIP(hipEventCreate(&res.a));
+    MHX_HIP(hipEventCreate(&res.b));
+    hipStream_t s = res.s;
+    hipEvent_t a = res.a, b = res.b;
     hipLaunchKernelGGL(k_mfma_layout, dim3(1), dim3(64), 0, s, lay.p);
     std::vector<double> hl(256);
     MHX_HIP(hipMemcpyAsync(hl.data(), lay.p, 256 * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -196,9 +209,6 @@ extern "C" int mhx_mfma_probe(int64_t n_elem, int32_t reps, int32_t variant, dou
     hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, s, (long)n_elem, d.p, sums.p, (long)n_full, full.p);
     std::vector<double> hf((size_t)n_full * 4 * 64 * 4);
     MHX_HIP(hipMemcpyAsync(hf.data(), full.p, hf.size() * sizeof(double), hipMemcpyDeviceToHost, s));
-    hipEvent_t a, b;
-    MHX_HIP(hipEventCreate(&a));
-    MHX_HIP(hipEventCreate(&b));
     MHX_HIP(hipEventRecord(a, s));
     for (int k = 0; k < reps; ++k)
       hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, s, (long)n_elem, d.p, sums.p, 0L, full.p);
@@ -210,9 +220,6 @@ extern "C" int mhx_mfma_probe(int64_t n_elem, int32_t reps, int32_t variant, dou
     std::vector<double> h(n_elem);
     MHX_HIP(hipMemcpyAsync(h.data(), sums.p, n_elem * sizeof(double), hipMemcpyDeviceToHost, s));
     MHX_HIP(hipStreamSynchronize(s));
-    (void)hipEventDestroy(a);
-    (void)hipEventDestroy(b);
-    (void)hipStreamDestroy(s);
     // the accumulator layout: every (row, column) of the 16 x 16 tile exactly once
     int row[256], colm[256], seen[256] = {0};
     for (int t = 0; t < 256; ++t) {
