@@ -686,7 +686,16 @@ __global__ __launch_bounds__(256) void k_mupdate(long n, int K, const T* __restr
   double v = 0.0;
   for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
     double wi = (double)w[i];
-    for (int k = 0; k < K; ++k) wi = fma(-h[k], (double)V[(long)k * ld + i], wi);
+    // eight basis loads in flight per thread, then the updates in basis order
+    int k = 0;
+    for (; k + 8 <= K; k += 8) {
+      double vk[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) vk[u] = (double)V[(long)(k + u) * ld + i];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) wi = fma(-h[k + u], vk[u], wi);
+    }
+    for (; k < K; ++k) wi = fma(-h[k], (double)V[(long)k * ld + i], wi);
     const T ws = (T)wi;
     w[i] = ws;
     v = fma((double)ws, (double)ws, v);
